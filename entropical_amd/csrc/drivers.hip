@@ -1,0 +1,284 @@
+// drivers.hip -- batched MI / transfer-entropy / entropy drivers (C ABI).
+//
+// Each driver builds the full job table of its reference driver loop, runs it
+// as ONE engine batch (one persistent density launch) and reduces the
+// density arrays on the device in FP64; only the per-result scalars return
+// to the host.
+#include <hip/hip_runtime.h>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "../../include/entropical_kde.h"
+#include "ctx.hpp"
+#include "engine.hpp"
+#include "reductions.hpp"
+
+namespace {
+
+using ent::set_error;
+
+#define DRV_HIP(call)                                                                \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      set_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #call);  \
+      return e_ == hipErrorOutOfMemory ? ENT_ERR_OOM : ENT_ERR_HIP;                  \
+    }                                                                                \
+  } while (0)
+
+#define DRV_TRY(expr)            \
+  do {                           \
+    int r_ = (expr);             \
+    if (r_ != ENT_OK) return r_; \
+  } while (0)
+
+int check_common(ent_ctx* ctx, const float* X, size_t n_rows, size_t n_cols, const float* h) {
+  if (!ctx || !ctx->eng) {
+    set_error("null context");
+    return ENT_ERR_ARG;
+  }
+  if (!X || !h || n_rows == 0 || n_cols == 0) {
+    set_error("null argument or empty input");
+    return ENT_ERR_ARG;
+  }
+  if (n_rows >= (1ull << 32)) {
+    set_error("more than 2^32 rows");
+    return ENT_ERR_UNSUPPORTED;
+  }
+  for (size_t c = 0; c < n_cols; ++c)
+    if (!(h[c] > 0.0f) || !std::isfinite(h[c])) {
+      set_error("error: bandwidths must be finite and positive");
+      return ENT_ERR_ARG;
+    }
+  return ENT_OK;
+}
+
+ent::JobSpec make_job(size_t n_rows, std::initializer_list<uint32_t> cols, std::initializer_list<float> hs,
+                      std::initializer_list<uint32_t> taus, uint64_t out_off) {
+  ent::JobSpec j;
+  j.d = (uint32_t)cols.size();
+  uint32_t tmax = 0;
+  for (uint32_t t : taus) tmax = t > tmax ? t : tmax;
+  j.n = (uint32_t)(n_rows - tmax);
+  j.n_rows = (uint32_t)n_rows;
+  j.out_off = out_off;
+  auto ci = cols.begin();
+  auto hi = hs.begin();
+  auto ti = taus.begin();
+  for (uint32_t e = 0; e < j.d; ++e, ++ci, ++hi, ++ti) {
+    j.off[e] = (uint64_t)(*ci) * n_rows + *ti;
+    j.h[e] = *hi;
+  }
+  return j;
+}
+
+// Densities of all jobs into ctx->out, then the reduction items; results
+// (one double per item) are copied to host `res`.
+int run_and_reduce(ent_ctx* ctx, const float* X, const std::vector<ent::JobSpec>& jobs, int method,
+                   const std::vector<ent::RedItem>& items, std::vector<double>& res) {
+  ent::Engine& E = *ctx->eng;
+  uint64_t total = 0;
+  for (const auto& j : jobs) total = std::max<uint64_t>(total, j.out_off + j.n);
+  DRV_TRY(E.ensure(ctx->out, sizeof(double) * total));
+  DRV_TRY(E.densities(X, jobs, (double*)ctx->out.p, method));
+  const uint32_t NI = (uint32_t)items.size();
+  uint64_t maxn = 1;
+  for (const auto& it : items) maxn = std::max<uint64_t>(maxn, it.n);
+  uint32_t bpi = (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, maxn / 16384));
+  size_t items_bytes = (sizeof(ent::RedItem) * NI + 255) / 256 * 256;
+  size_t part_bytes = sizeof(double) * (size_t)NI * bpi;
+  DRV_TRY(E.ensure(ctx->red, items_bytes + part_bytes + sizeof(double) * NI));
+  char* base = (char*)ctx->red.p;
+  ent::RedItem* d_items = (ent::RedItem*)base;
+  double* d_part = (double*)(base + items_bytes);
+  double* d_out = (double*)(base + items_bytes + part_bytes);
+  DRV_TRY(E.ensure_host(sizeof(ent::RedItem) * NI + sizeof(double) * NI));
+  char* hs = (char*)E.host_stage();
+  std::memcpy(hs, items.data(), sizeof(ent::RedItem) * NI);
+  double* h_out = (double*)(hs + sizeof(ent::RedItem) * NI);
+  DRV_HIP(hipMemcpyAsync(d_items, hs, sizeof(ent::RedItem) * NI, hipMemcpyHostToDevice, E.stream()));
+  ent::launch_reduce(E.stream(), d_items, NI, (const double*)ctx->out.p, d_part, bpi, d_out);
+  DRV_HIP(hipGetLastError());
+  DRV_HIP(hipMemcpyAsync(h_out, d_out, sizeof(double) * NI, hipMemcpyDeviceToHost, E.stream()));
+  DRV_HIP(hipStreamSynchronize(E.stream()));
+  res.assign(h_out, h_out + NI);
+  return ENT_OK;
+}
+
+ent::RedItem item(uint32_t op, std::initializer_list<uint64_t> offs, uint64_t n, double pref, double scale) {
+  ent::RedItem it;
+  std::memset(&it, 0, sizeof(it));
+  it.op = op;
+  int e = 0;
+  for (uint64_t o : offs) it.off[e++] = o;
+  it.n = n;
+  it.pref = pref;
+  it.scale = scale;
+  return it;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ent_mi_matrix_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
+                      int bits, int method, double* mi) {
+  DRV_TRY(check_common(ctx, X_dev, n_rows, n_cols, h));
+  if (!mi) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  const size_t C = n_cols, N = n_rows;
+  const double sc = bits ? 1.0 / std::log(2.0) : 1.0;
+  std::vector<ent::JobSpec> jobs;
+  std::vector<ent::RedItem> items;
+  // 2-D jobs first (the larger ones), then the 1-D marginals computed once
+  // per column (mi.cpp:38-46 recomputes p_j for every pair).
+  const uint64_t marg0 = 0;
+  uint64_t slot = C;
+  for (uint32_t i = 0; i < C; ++i)
+    for (uint32_t j = i + 1; j < C; ++j) {
+      jobs.push_back(make_job(N, {i, j}, {h[i], h[j]}, {0, 0}, slot * N));
+      items.push_back(item(ent::kRedMI, {slot * N, (marg0 + i) * N, (marg0 + j) * N}, N, 0.0, sc));
+      ++slot;
+    }
+  for (uint32_t c = 0; c < C; ++c) jobs.push_back(make_job(N, {c}, {h[c]}, {0}, (marg0 + c) * N));
+  std::vector<double> res;
+  DRV_TRY(run_and_reduce(ctx, X_dev, jobs, method, items, res));
+  size_t p = 0;
+  for (size_t i = 0; i < C; ++i) mi[i * C + i] = 0.0;
+  for (size_t i = 0; i < C; ++i)
+    for (size_t j = i + 1; j < C; ++j, ++p) {
+      mi[i * C + j] = res[p];
+      mi[j * C + i] = res[p];
+    }
+  ctx->driver_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return ENT_OK;
+}
+
+int ent_te_matrix_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
+                      const uint32_t* taus, size_t n_taus, int bits, int method, double* te) {
+  DRV_TRY(check_common(ctx, X_dev, n_rows, n_cols, h));
+  if (!te || !taus || n_taus == 0) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  for (size_t t = 0; t < n_taus; ++t)
+    if (taus[t] >= n_rows) {
+      set_error("error: lag time (tau) must be smaller than the number of rows");
+      return ENT_ERR_ARG;
+    }
+  auto t0 = std::chrono::steady_clock::now();
+  const size_t C = n_cols, N = n_rows;
+  const double sc = bits ? 1.0 / std::log(2.0) : 1.0;
+  std::vector<ent::JobSpec> jobs;
+  std::vector<ent::RedItem> items;
+  uint64_t next = 0;
+  auto alloc = [&](uint64_t n) {
+    uint64_t o = next;
+    next += n;
+    return o;
+  };
+  // transs.cpp:42-74 job set.  3-D jobs first (largest), then the rest.
+  std::vector<uint64_t> off_p(C), off_pij(C * C, 0);
+  std::vector<std::vector<uint64_t>> off_ptau(n_taus, std::vector<uint64_t>(C));
+  std::vector<std::vector<uint64_t>> off_itau(n_taus, std::vector<uint64_t>(C * C)), off_jtau = off_itau;
+  for (size_t t = 0; t < n_taus; ++t) {
+    const uint32_t tau = taus[t];
+    for (uint32_t i = 0; i < C; ++i)
+      for (uint32_t j = i + 1; j < C; ++j) {
+        off_itau[t][i * C + j] = alloc(N - tau);
+        jobs.push_back(make_job(N, {i, i, j}, {h[i], h[i], h[j]}, {0, tau, 0}, off_itau[t][i * C + j]));
+        off_jtau[t][i * C + j] = alloc(N - tau);
+        jobs.push_back(make_job(N, {i, j, j}, {h[i], h[j], h[j]}, {0, 0, tau}, off_jtau[t][i * C + j]));
+      }
+  }
+  for (uint32_t i = 0; i < C; ++i)
+    for (uint32_t j = i + 1; j < C; ++j) {
+      off_pij[i * C + j] = alloc(N);
+      jobs.push_back(make_job(N, {i, j}, {h[i], h[j]}, {0, 0}, off_pij[i * C + j]));
+    }
+  for (size_t t = 0; t < n_taus; ++t)
+    for (uint32_t i = 0; i < C; ++i) {
+      off_ptau[t][i] = alloc(N - taus[t]);
+      jobs.push_back(make_job(N, {i, i}, {h[i], h[i]}, {0, taus[t]}, off_ptau[t][i]));
+    }
+  for (uint32_t i = 0; i < C; ++i) {
+    off_p[i] = alloc(N);
+    jobs.push_back(make_job(N, {i}, {h[i]}, {0}, off_p[i]));
+  }
+  // reduction items: per tau, per pair, both directions (transs.cpp:75-113)
+  for (size_t t = 0; t < n_taus; ++t) {
+    const uint64_t n = N - taus[t];
+    for (uint32_t i = 0; i < C; ++i)
+      for (uint32_t j = i + 1; j < C; ++j) {
+        const double hi = h[i], hj = h[j];
+        items.push_back(item(ent::kRedTE, {off_itau[t][i * C + j], off_p[i], off_pij[i * C + j], off_ptau[t][i]},
+                             n, hi * hi * hj, sc));
+        items.push_back(item(ent::kRedTE, {off_jtau[t][i * C + j], off_p[j], off_pij[i * C + j], off_ptau[t][j]},
+                             n, hi * hj * hi, sc));
+      }
+  }
+  std::vector<double> res;
+  DRV_TRY(run_and_reduce(ctx, X_dev, jobs, method, items, res));
+  size_t p = 0;
+  for (size_t t = 0; t < n_taus; ++t) {
+    double* M = te + t * C * C;
+    for (size_t k = 0; k < C * C; ++k) M[k] = 0.0;
+    for (size_t i = 0; i < C; ++i)
+      for (size_t j = i + 1; j < C; ++j) {
+        M[i * C + j] = res[p++];
+        M[j * C + i] = res[p++];
+      }
+  }
+  ctx->driver_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return ENT_OK;
+}
+
+int ent_batch_entropy_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs, int method,
+                          double* H) {
+  if (!ctx || !ctx->eng || !X_dev || !jobs || !H) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<ent::JobSpec> js(n_jobs);
+  std::vector<ent::RedItem> items(n_jobs);
+  uint64_t next = 0;
+  for (size_t j = 0; j < n_jobs; ++j) {
+    const ent_job& a = jobs[j];
+    if (a.n_dim < 1 || a.n_dim > 3 || a.n == 0 || a.n_rows == 0) {
+      set_error("invalid job: n_dim must be 1..3 and n, n_rows > 0");
+      return ENT_ERR_ARG;
+    }
+    js[j].d = a.n_dim;
+    js[j].n = a.n;
+    js[j].n_rows = a.n_rows;
+    js[j].out_off = next;
+    for (uint32_t e = 0; e < 3; ++e) {
+      js[j].off[e] = a.off[e];
+      js[j].h[e] = e < a.n_dim ? a.h[e] : 1.0f;
+    }
+    items[j] = item(ent::kRedLogMean, {next}, a.n, 0.0, 1.0);
+    next += a.n;
+  }
+  std::vector<double> res;
+  DRV_TRY(run_and_reduce(ctx, X_dev, js, method, items, res));
+  for (size_t j = 0; j < n_jobs; ++j) H[j] = -res[j] / (double)jobs[j].n;
+  ctx->driver_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return ENT_OK;
+}
+
+int ent_ctx_last_driver_ms(ent_ctx* ctx, double* ms) {
+  if (!ctx || !ms) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  *ms = ctx->driver_ms;
+  return ENT_OK;
+}
+
+}  // extern "C"

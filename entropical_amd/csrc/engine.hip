@@ -1,0 +1,356 @@
+// engine.hip -- host side of the KDE engine: workspace, planning, launches.
+#include "engine.hpp"
+
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include "kde_kernels.hpp"
+
+namespace ent {
+
+namespace {
+thread_local std::string g_last_error;
+
+inline float o2f(uint32_t o) {
+  uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+}  // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+#define ENT_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      set_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #call);     \
+      return e_ == hipErrorOutOfMemory ? kErrOom : kErrHip;                             \
+    }                                                                                   \
+  } while (0)
+
+#define ENT_TRY(expr)              \
+  do {                             \
+    int r_ = (expr);               \
+    if (r_ != kOk) return r_;      \
+  } while (0)
+
+Engine::Engine(int device) : device_(device) {
+  if (hipSetDevice(device) != hipSuccess) return;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) num_cus_ = prop.multiProcessorCount;
+  hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
+  for (auto& e : ev_) hipEventCreate(&e);
+}
+
+Engine::~Engine() {
+  hipSetDevice(device_);
+  if (stream_) hipStreamSynchronize(stream_);
+  DevBuf* bufs[] = {&d_jobs,  &d_chunks, &d_chunk_first, &d_mm,     &d_flags,        &d_keys[0],
+                    &d_keys[1], &d_idx[0], &d_idx[1],     &d_hist,   &d_scan_scratch, &d_scoord,
+                    &d_tile_cnt, &d_tiles, &d_counter};
+  for (DevBuf* b : bufs)
+    if (b->p) hipFree(b->p);
+  for (auto& e : ev_)
+    if (e) hipEventDestroy(e);
+  if (host_stage_) hipHostFree(host_stage_);
+  if (stream_) hipStreamDestroy(stream_);
+}
+
+int Engine::ensure(DevBuf& b, size_t bytes) {
+  if (bytes <= b.cap) return kOk;
+  ENT_HIP(hipStreamSynchronize(stream_));
+  if (b.p) ENT_HIP(hipFree(b.p));
+  b.p = nullptr;
+  b.cap = 0;
+  size_t want = std::max<size_t>(bytes + bytes / 8, 4096);
+  ENT_HIP(hipMalloc(&b.p, want));
+  b.cap = want;
+  return kOk;
+}
+
+int Engine::ensure_host(size_t bytes) {
+  if (bytes <= host_stage_cap_) return kOk;
+  ENT_HIP(hipStreamSynchronize(stream_));
+  if (host_stage_) ENT_HIP(hipHostFree(host_stage_));
+  host_stage_ = nullptr;
+  host_stage_cap_ = 0;
+  size_t want = std::max<size_t>(bytes, 1 << 20);
+  ENT_HIP(hipHostMalloc(&host_stage_, want, hipHostMallocDefault));
+  host_stage_cap_ = want;
+  return kOk;
+}
+
+int Engine::densities(const float* X, const std::vector<JobSpec>& jobs, double* dens, int method,
+                      int stop) {
+  if (!stream_) {
+    set_error("engine has no stream (device initialisation failed)");
+    return kErrNoDevice;
+  }
+  ENT_HIP(hipSetDevice(device_));
+  last = RunStats{};
+  if (jobs.empty()) return kOk;
+  for (const JobSpec& j : jobs) {
+    if (j.d < 1 || j.d > 3) {
+      set_error("job dimension must be 1, 2 or 3");
+      return kErrArg;
+    }
+    if (j.n == 0) {
+      set_error("job with zero points");
+      return kErrArg;
+    }
+    for (uint32_t e = 0; e < j.d; ++e)
+      if (!(j.h[e] > 0.0f) || !std::isfinite(j.h[e])) {
+        set_error("bandwidths must be finite and > 0");
+        return kErrArg;
+      }
+  }
+  // host job table
+  h_jobs.assign(jobs.size(), DJob{});
+  h_chunks.clear();
+  h_chunk_first.assign(jobs.size() + 1, 0);
+  uint64_t pbase = 0;
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const JobSpec& s = jobs[j];
+    DJob& J = h_jobs[j];
+    std::memset(&J, 0, sizeof(J));
+    J.d = s.d;
+    J.n = s.n;
+    J.n_rows = s.n_rows;
+    J.flags = (method == kMethodBrute ? kJobBrute : 0u) | s.flags;
+    if ((s.flags & kJobTri) && s.d != 1) {
+      set_error("triangular-kernel jobs must be 1-D");
+      return kErrArg;
+    }
+    J.pbase = pbase;
+    J.out_off = s.out_off;
+    double norm = 1.0 / (double)s.n_rows;
+    for (uint32_t e = 0; e < 3; ++e) {
+      J.off[e] = e < s.d ? s.off[e] : 0;
+      J.h[e] = e < s.d ? s.h[e] : 1.0f;
+      J.hinv[e] = 1.0f / J.h[e];
+      J.nb[e] = 1;
+      J.bs[e] = 1.0f;
+      if (e < s.d) norm *= 0.75 / (double)s.h[e];
+    }
+    J.norm = s.norm != 0.0 ? s.norm : norm;
+    h_chunk_first[j] = (uint32_t)h_chunks.size();
+    for (uint32_t st = 0; st < s.n; st += kChunk) h_chunks.push_back(make_uint2((uint32_t)j, st));
+    pbase += s.n;
+  }
+  h_chunk_first[jobs.size()] = (uint32_t)h_chunks.size();
+  total_points = pbase;
+  if (total_points >= (1ull << 32)) {
+    set_error("batch exceeds 2^32 points");
+    return kErrUnsupported;
+  }
+  if (method == kMethodBrute) return run_brute(X, jobs, dens);
+  return plan_and_run_cell(X, jobs, dens, stop);
+}
+
+int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, double* dens,
+                              int stop) {
+  const uint32_t J = (uint32_t)jobs.size();
+  const uint32_t NC = (uint32_t)h_chunks.size();
+  const uint64_t P = total_points;
+  ENT_TRY(ensure(d_jobs, sizeof(DJob) * J));
+  ENT_TRY(ensure(d_chunks, sizeof(uint2) * NC));
+  ENT_TRY(ensure(d_chunk_first, sizeof(uint32_t) * (J + 1)));
+  ENT_TRY(ensure(d_mm, sizeof(uint32_t) * 6 * J));
+  ENT_TRY(ensure(d_flags, 64));
+  // host staging: jobs + chunks + chunk_first + minmax readback
+  size_t stage_bytes = sizeof(DJob) * J + sizeof(uint2) * NC + sizeof(uint32_t) * (J + 1) +
+                       sizeof(uint32_t) * 6 * J + 64;
+  ENT_TRY(ensure_host(stage_bytes));
+  char* hs = (char*)host_stage_;
+  DJob* hs_jobs = (DJob*)hs;
+  uint2* hs_chunks = (uint2*)(hs_jobs + J);
+  uint32_t* hs_cf = (uint32_t*)(hs_chunks + NC);
+  uint32_t* hs_mm = hs_cf + (J + 1);
+  uint32_t* hs_flags = hs_mm + 6 * J;
+  std::memcpy(hs_jobs, h_jobs.data(), sizeof(DJob) * J);
+  std::memcpy(hs_chunks, h_chunks.data(), sizeof(uint2) * NC);
+  std::memcpy(hs_cf, h_chunk_first.data(), sizeof(uint32_t) * (J + 1));
+
+  ENT_HIP(hipEventRecord(ev_[0], stream_));
+  ENT_HIP(hipMemcpyAsync(d_jobs.p, hs_jobs, sizeof(DJob) * J, hipMemcpyHostToDevice, stream_));
+  ENT_HIP(hipMemcpyAsync(d_chunks.p, hs_chunks, sizeof(uint2) * NC, hipMemcpyHostToDevice, stream_));
+  ENT_HIP(hipMemcpyAsync(d_chunk_first.p, hs_cf, sizeof(uint32_t) * (J + 1), hipMemcpyHostToDevice, stream_));
+  const DJob* dj = (const DJob*)d_jobs.p;
+  const uint2* dch = (const uint2*)d_chunks.p;
+  const uint32_t* dcf = (const uint32_t*)d_chunk_first.p;
+  launch_init_minmax(stream_, (uint32_t*)d_mm.p, J);
+  launch_fill_u32(stream_, (uint32_t*)d_flags.p, 0u, 16);
+  launch_minmax(stream_, dj, dch, NC, X, (uint32_t*)d_mm.p, (uint32_t*)d_flags.p);
+  ENT_HIP(hipGetLastError());
+  ENT_HIP(hipMemcpyAsync(hs_mm, d_mm.p, sizeof(uint32_t) * 6 * J, hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipMemcpyAsync(hs_flags, d_flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipStreamSynchronize(stream_));
+  if (hs_flags[0] != 0) {
+    set_error("non-finite coordinate in density input");
+    return kErrNonFinite;
+  }
+  // host planning: boxes per dim (boxedsearch.cpp:21-27 with the total fixes)
+  uint64_t max_key = 0;
+  for (uint32_t j = 0; j < J; ++j) {
+    DJob& D = h_jobs[j];
+    uint64_t cells = 1;
+    for (uint32_t e = 0; e < D.d; ++e) {
+      float mn = o2f(hs_mm[j * 6 + e * 2]);
+      float mx = o2f(hs_mm[j * 6 + e * 2 + 1]);
+      float m = (float)(e == 0 ? tuning.m0 : tuning.mr);
+      volatile float bs = D.h[e] / m;
+      volatile float range = mx - mn;
+      volatile float q = range / bs;
+      float nbf = std::ceil((float)q);
+      if (!(nbf < 16777216.0f)) {
+        set_error("bandwidth too small for the data range (more than 2^24 boxes per dim)");
+        return kErrUnsupported;
+      }
+      uint32_t nb = nbf < 1.0f ? 1u : (uint32_t)nbf;
+      D.mn[e] = mn;
+      D.bs[e] = bs;
+      D.nb[e] = nb;
+      cells *= nb;
+      if (cells >= (1ull << 62)) {
+        set_error("too many cells for the bandwidths given");
+        return kErrUnsupported;
+      }
+    }
+    D.group = (uint32_t)std::max(1, tuning.group_h * tuning.m0);
+    max_key = std::max<uint64_t>(max_key, cells - 1);
+  }
+  int bits = 0;
+  while (bits < 64 && (max_key >> bits) != 0) ++bits;
+  const int npass = (bits + kRadixBits - 1) / kRadixBits;
+  last.radix_passes = (uint32_t)npass;
+  std::memcpy(hs_jobs, h_jobs.data(), sizeof(DJob) * J);
+  ENT_HIP(hipMemcpyAsync(d_jobs.p, hs_jobs, sizeof(DJob) * J, hipMemcpyHostToDevice, stream_));
+  if (stop == kStopAfterPlan) return kOk;
+
+  // buffers sized by the plan
+  const uint64_t HM = (uint64_t)kRadixBins * NC;
+  const uint64_t scan_m = std::max<uint64_t>(HM, NC);
+  ENT_TRY(ensure(d_keys[0], sizeof(uint64_t) * P));
+  ENT_TRY(ensure(d_keys[1], sizeof(uint64_t) * P));
+  ENT_TRY(ensure(d_idx[0], sizeof(uint32_t) * P));
+  ENT_TRY(ensure(d_idx[1], sizeof(uint32_t) * P));
+  ENT_TRY(ensure(d_hist, sizeof(uint32_t) * (HM + 1)));
+  ENT_TRY(ensure(d_scan_scratch, sizeof(uint32_t) * scan_scratch_elems(scan_m)));
+  ENT_TRY(ensure(d_scoord, sizeof(float) * 3 * P));
+  ENT_TRY(ensure(d_tile_cnt, sizeof(uint32_t) * (NC + 1)));
+  ENT_TRY(ensure(d_tiles, sizeof(DTile) * P));
+  ENT_TRY(ensure(d_counter, 64));
+
+  uint64_t* keys[2] = {(uint64_t*)d_keys[0].p, (uint64_t*)d_keys[1].p};
+  uint32_t* idx[2] = {(uint32_t*)d_idx[0].p, (uint32_t*)d_idx[1].p};
+  uint32_t* hist = (uint32_t*)d_hist.p;
+  uint32_t* scratch = (uint32_t*)d_scan_scratch.p;
+
+  launch_cell_keys(stream_, dj, dch, NC, X, keys[0], idx[0]);
+  int cur = 0;
+  for (int pass = 0; pass < npass; ++pass) {
+    int shift = pass * kRadixBits;
+    launch_radix_hist(stream_, dj, dch, dcf, NC, keys[cur], shift, hist);
+    launch_exclusive_scan(stream_, hist, hist, HM, scratch);
+    launch_radix_scatter(stream_, dj, dch, dcf, NC, keys[cur], idx[cur], keys[cur ^ 1], idx[cur ^ 1],
+                         shift, hist);
+    cur ^= 1;
+  }
+  sorted_buf = cur;
+  if (stop == kStopAfterSort) {
+    ENT_HIP(hipGetLastError());
+    return kOk;
+  }
+  launch_gather_sorted(stream_, dj, dch, NC, X, idx[cur], (float*)d_scoord.p);
+  const uint32_t tq = (uint32_t)(kWave * tuning.q);
+  uint32_t* tcnt = (uint32_t*)d_tile_cnt.p;
+  launch_tile_count(stream_, dj, dch, NC, keys[cur], tq, tcnt);
+  launch_exclusive_scan(stream_, tcnt, tcnt, NC, scratch);
+  launch_tile_write(stream_, dj, dch, NC, keys[cur], tq, tcnt, (DTile*)d_tiles.p);
+  launch_tile_ends(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
+  ENT_HIP(hipGetLastError());
+  return launch_density_common(false, X, dens, P);
+}
+
+int Engine::run_brute(const float* X, const std::vector<JobSpec>& jobs, double* dens) {
+  const uint32_t J = (uint32_t)jobs.size();
+  const uint32_t tq = (uint32_t)(kWave * tuning.q);
+  std::vector<DTile> tiles;
+  for (uint32_t j = 0; j < J; ++j)
+    for (uint32_t p = 0; p < jobs[j].n; p += tq) {
+      DTile t;
+      t.job = j;
+      t.p0 = p;
+      t.p1 = std::min(jobs[j].n, p + tq);
+      t.pad = 0;
+      tiles.push_back(t);
+    }
+  const uint64_t T = tiles.size();
+  ENT_TRY(ensure(d_jobs, sizeof(DJob) * J));
+  ENT_TRY(ensure(d_tiles, sizeof(DTile) * T));
+  ENT_TRY(ensure(d_tile_cnt, sizeof(uint32_t) * 2));
+  ENT_TRY(ensure(d_counter, 64));
+  size_t stage_bytes = sizeof(DJob) * J + sizeof(DTile) * T + 16;
+  ENT_TRY(ensure_host(stage_bytes));
+  char* hs = (char*)host_stage_;
+  std::memcpy(hs, h_jobs.data(), sizeof(DJob) * J);
+  std::memcpy(hs + sizeof(DJob) * J, tiles.data(), sizeof(DTile) * T);
+  uint32_t* hs_T = (uint32_t*)(hs + sizeof(DJob) * J + sizeof(DTile) * T);
+  hs_T[0] = (uint32_t)T;
+  ENT_HIP(hipEventRecord(ev_[0], stream_));
+  ENT_HIP(hipMemcpyAsync(d_jobs.p, hs, sizeof(DJob) * J, hipMemcpyHostToDevice, stream_));
+  ENT_HIP(hipMemcpyAsync(d_tiles.p, hs + sizeof(DJob) * J, sizeof(DTile) * T, hipMemcpyHostToDevice, stream_));
+  ENT_HIP(hipMemcpyAsync(d_tile_cnt.p, hs_T, sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+  // the density kernel reads the tile count from n_tiles
+  return launch_density_common(true, X, dens, T);
+}
+
+int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64_t max_tiles) {
+  (void)max_tiles;
+  int bpc = 0;
+  if (density_occupancy(tuning.q, raw, &bpc) != 0 || bpc <= 0) {
+    set_error("unsupported queries-per-lane setting");
+    return kErrUnsupported;
+  }
+  unsigned grid = (unsigned)std::max(1, num_cus_ * bpc);
+  const uint32_t NC = (uint32_t)h_chunks.size();
+  const uint32_t* n_tiles = raw ? (const uint32_t*)d_tile_cnt.p : (const uint32_t*)d_tile_cnt.p + NC;
+  uint32_t* counter = (uint32_t*)d_counter.p;
+  launch_fill_u32(stream_, counter, 0u, 16);
+  ENT_HIP(hipEventRecord(ev_[1], stream_));
+  int rc = launch_density(stream_, tuning.q, raw, grid, (const DJob*)d_jobs.p, (const DTile*)d_tiles.p,
+                          n_tiles, counter, X, raw ? nullptr : (const float*)d_scoord.p,
+                          raw ? nullptr : (const uint64_t*)d_keys[sorted_buf].p,
+                          raw ? nullptr : (const uint32_t*)d_idx[sorted_buf].p, dens,
+                          (unsigned long long*)(counter + 2));
+  if (rc != 0) {
+    set_error("density launch: unsupported configuration");
+    return kErrUnsupported;
+  }
+  ENT_HIP(hipGetLastError());
+  ENT_HIP(hipEventRecord(ev_[2], stream_));
+  // stats readback
+  ENT_TRY(ensure_host(64));
+  uint32_t* hc = (uint32_t*)host_stage_;
+  ENT_HIP(hipMemcpyAsync(hc, counter, 32, hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipMemcpyAsync(hc + 8, n_tiles, 4, hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipStreamSynchronize(stream_));
+  uint64_t pd[3];
+  std::memcpy(pd, hc + 2, 24);
+  last.pairs = pd[0] + pd[1] + pd[2];
+  for (int d = 0; d < 3; ++d) last.pairs_by_d[d] = pd[d];
+  last.tiles = hc[8];
+  float ms = 0.f;
+  ENT_HIP(hipEventElapsedTime(&ms, ev_[1], ev_[2]));
+  last.density_ms = ms;
+  ENT_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[2]));
+  last.total_ms = ms;
+  return kOk;
+}
+
+}  // namespace ent

@@ -1,0 +1,102 @@
+// engine.hpp -- host side of the MI355X KDE engine (one context per device).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "kde_engine.hpp"
+
+namespace ent {
+
+enum Status : int {
+  kOk = 0,
+  kErrArg = 1,
+  kErrNoDevice = 2,
+  kErrHip = 3,
+  kErrOom = 4,
+  kErrNonFinite = 5,
+  kErrUnsupported = 6,
+};
+
+void set_error(const std::string& msg);
+const char* last_error();
+
+// One density estimate.  Point k, dim e: X[off[e] + k], k in [0, n).
+struct JobSpec {
+  uint32_t d = 1;
+  uint64_t off[3] = {0, 0, 0};
+  float h[3] = {1.f, 1.f, 1.f};
+  uint32_t n = 0;       // points
+  uint32_t n_rows = 0;  // normalisation count (original rows)
+  uint64_t out_off = 0; // first slot of this job's densities in the output
+  uint32_t flags = 0;   // kJobTri: triangular kernel (epa self-convolution)
+  double norm = 0.0;    // 0: prod_e(0.75/h_e)/n_rows; else this factor
+};
+
+enum Stop : int { kStopNone = 0, kStopAfterPlan = 1, kStopAfterSort = 2 };
+
+enum Method : int { kMethodCell = 0, kMethodBrute = 1 };
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct RunStats {
+  double density_ms = 0.0;   // persistent density launch (HIP events)
+  double total_ms = 0.0;     // whole pipeline on the stream (HIP events)
+  double plan_ms = 0.0;      // minmax + host planning round trip
+  uint64_t pairs = 0;        // executed (query, candidate) evaluations
+  uint64_t pairs_by_d[3] = {0, 0, 0};  // the same, per job dimension
+  uint64_t tiles = 0;
+  uint32_t radix_passes = 0;
+};
+
+class Engine {
+ public:
+  explicit Engine(int device);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  int num_cus() const { return num_cus_; }
+
+  // Densities of every job over device-resident X into device dens.
+  int densities(const float* X, const std::vector<JobSpec>& jobs, double* dens, int method,
+                int stop = kStopNone);
+
+  Tuning tuning;
+  RunStats last;
+
+  // grow-only device workspace
+  int ensure(DevBuf& b, size_t bytes);
+  int ensure_host(size_t bytes);
+  void* host_stage() { return host_stage_; }
+
+ private:
+  int plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, double* dens, int stop);
+  int run_brute(const float* X, const std::vector<JobSpec>& jobs, double* dens);
+  int launch_density_common(bool raw, const float* X, double* dens, uint64_t max_tiles);
+
+  int device_ = 0;
+  int num_cus_ = 0;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev_[6] = {};
+  void* host_stage_ = nullptr;
+  size_t host_stage_cap_ = 0;
+
+ public:
+  // workspace buffers
+  DevBuf d_jobs, d_chunks, d_chunk_first, d_mm, d_flags, d_keys[2], d_idx[2], d_hist,
+      d_scan_scratch, d_scoord, d_tile_cnt, d_tiles, d_counter;
+  // host copies
+  std::vector<DJob> h_jobs;
+  std::vector<uint2> h_chunks;
+  std::vector<uint32_t> h_chunk_first;
+  uint64_t total_points = 0;
+  int sorted_buf = 0;  // which of d_keys/d_idx holds the sorted result
+};
+
+}  // namespace ent

@@ -1,0 +1,67 @@
+// kde_engine.hpp -- shared host/device declarations of the MI355X KDE engine.
+//
+// Density definition (densities_omp.cpp:19-26,87-93; density_1d.cuh:39-46;
+// densities_cuda.cu:95-103):
+//   p(x_i) = 1/n_rows * sum_j prod_e K_{h_e}(x_{i,e} - x_{j,e}),
+//   K_h(u) = 0.75/h * (1 - (u/h)^2) for |u| <= h, else 0,
+// over every point of the job including the self pair, divided by the
+// ORIGINAL row count even for lagged jobs.
+//
+// Engine stages for a batch of jobs (one stream, capturable in a hipGraph):
+//   minmax -> (host: boxes per dim) -> cell keys -> stable LSD radix sort by
+//   cell key -> gather sorted coordinates -> query tiles -> one persistent
+//   density launch over every (job, tile) -> reductions.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+namespace ent {
+
+constexpr int kWave = 64;          // CDNA wavefront
+constexpr int kBlock = 256;        // 4 waves per workgroup
+constexpr int kChunkItems = 32;    // items per lane in chunked passes
+constexpr int kChunk = kWave * kChunkItems;  // points per chunk (one wave)
+constexpr int kRadixBits = 8;
+constexpr int kRadixBins = 1 << kRadixBits;
+constexpr int kMaxRanges = 64;     // candidate row ranges per tile (one per lane)
+
+// Job flags
+constexpr uint32_t kJobBrute = 1u;  // all-pairs over the job's points (plain path)
+constexpr uint32_t kJobTri = 2u;    // 1-D triangular kernel max(0, 1-|u|): the
+                                    // Epanechnikov self-convolution of epa_convolution
+
+// Device job descriptor.  Points of job j, dim e, index k in [0, n):
+//   X[off[e] + k].
+struct DJob {
+  uint64_t off[3];
+  float h[3];
+  float hinv[3];
+  float mn[3];       // bin origin (min of the job's points, dim e)
+  float bs[3];       // box size, dim e
+  uint32_t nb[3];    // boxes per dim (>= 1)
+  uint32_t d;
+  uint32_t n;        // points
+  uint32_t n_rows;   // normalisation count
+  uint32_t flags;
+  uint32_t group;    // dim-0 boxes per tile group (tile extent bound)
+  uint64_t pbase;    // first slot in the concatenated point arrays
+  uint64_t out_off;  // first density slot in the output array
+  double norm;       // prod_e(0.75/h_e) / n_rows
+};
+
+// Tile: queries [p0, p1) of job `job` (sorted positions, job-local).
+struct DTile {
+  uint32_t job;
+  uint32_t p0;
+  uint32_t p1;
+  uint32_t pad;
+};
+
+struct Tuning {
+  int q = 4;            // queries per lane (tile = 64*q queries)
+  int m0 = 4;           // dim-0 boxes per bandwidth
+  int mr = 2;           // row-dim boxes per bandwidth
+  int group_h = 2;      // tile extent bound along dim 0, in bandwidths
+};
+
+}  // namespace ent

@@ -1,0 +1,88 @@
+// reductions.hip -- FP64 reductions of density arrays into per-result scalars.
+//
+// One "item" reduces one formula over k in [0, n) of up to four density
+// arrays (device doubles).  Two deterministic stages: fixed-size partial sums
+// per (item, block), then one ordered sum per item.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+#include "reductions.hpp"
+
+namespace ent {
+namespace {
+
+constexpr int kRBlock = 256;
+
+__device__ __forceinline__ double inv0(double x) { return x == 0.0 ? x : 1.0 / x; }  // tools.hxx:45-53
+
+// One term of the selected formula (FP64 throughout; log of 0 -> 0 as
+// select_log does, tools.cpp:18-37).
+__device__ __forceinline__ double term(const RedItem& it, uint64_t k, const double* __restrict__ D) {
+  const double a = D[it.off[0] + k];
+  switch (it.op) {
+    case kRedMI: {  // mi.cpp:53-56: p_ij * log(p_ij / p_i / p_j)
+      const double r = a / D[it.off[1] + k] / D[it.off[2] + k];
+      return r == 0.0 ? 0.0 : a * log(r) * it.scale;
+    }
+    case kRedLogMean:  // sum of ln p (entropy = -mean)
+      return log(a);
+    case kRedTE: {  // transs.cpp:77-100: p3*pref*log(p3*p1*inv(p2)*inv(ptau))
+      if (!(a > 0.0)) return 0.0;
+      const double r = a * D[it.off[1] + k] * inv0(D[it.off[2] + k]) * inv0(D[it.off[3] + k]);
+      if (r == 0.0) return 0.0;
+      const double te = a * it.pref * log(r) * it.scale;
+      return (isfinite(te) && a <= 1.0) ? te : 0.0;
+    }
+    case kRedAutoEnt: {  // autoent.cpp:52-56: p2*h*h*log(p1*inv(p2*h)), p2 = a
+      if (!(a > 0.0)) return 0.0;
+      const double r = D[it.off[1] + k] * inv0(a * it.pref);
+      return r == 0.0 ? 0.0 : a * it.pref * it.pref * log(r) * it.scale;
+    }
+    default:
+      return 0.0;
+  }
+}
+
+__global__ __launch_bounds__(kRBlock) void k_reduce_partial(const RedItem* __restrict__ items,
+                                                            const double* __restrict__ D,
+                                                            double* __restrict__ partial,
+                                                            uint32_t blocks_per_item) {
+  __shared__ double s[kRBlock];
+  const uint32_t item = blockIdx.y;
+  const RedItem it = items[item];
+  const uint64_t n = it.n;
+  const uint64_t per = (n + blocks_per_item - 1) / blocks_per_item;
+  const uint64_t k0 = (uint64_t)blockIdx.x * per;
+  const uint64_t k1 = k0 + per < n ? k0 + per : n;
+  double acc = 0.0;
+  for (uint64_t k = k0 + threadIdx.x; k < k1; k += kRBlock) acc += term(it, k, D);
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = kRBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(uint64_t)item * blocks_per_item + blockIdx.x] = s[0];
+}
+
+__global__ void k_reduce_final(const double* __restrict__ partial, uint32_t blocks_per_item,
+                               uint32_t n_items, double* __restrict__ out) {
+  const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= n_items) return;
+  double acc = 0.0;
+  for (uint32_t b = 0; b < blocks_per_item; ++b) acc += partial[(uint64_t)item * blocks_per_item + b];
+  out[item] = acc;
+}
+
+}  // namespace
+
+void launch_reduce(hipStream_t s, const RedItem* items, uint32_t n_items, const double* D,
+                   double* partial, uint32_t blocks_per_item, double* out) {
+  if (n_items == 0) return;
+  hipLaunchKernelGGL(k_reduce_partial, dim3(blocks_per_item, n_items), dim3(kRBlock), 0, s, items, D,
+                     partial, blocks_per_item);
+  hipLaunchKernelGGL(k_reduce_final, dim3((n_items + 255) / 256), dim3(256), 0, s, partial,
+                     blocks_per_item, n_items, out);
+}
+
+}  // namespace ent

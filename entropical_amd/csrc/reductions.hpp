@@ -1,0 +1,28 @@
+// reductions.hpp -- per-result FP64 reductions over device density arrays.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ent {
+
+enum RedOp : uint32_t {
+  kRedMI = 0,       // sum p_ij log(p_ij / p_i / p_j)           (mi.cpp:53-56)
+  kRedLogMean = 1,  // sum ln p                                 (H = -mean ln p)
+  kRedTE = 2,       // sum p3 pref log(p3 p1 inv(p2) inv(ptau))  (transs.cpp:77-100)
+  kRedAutoEnt = 3,  // sum p2 h h log(p1 inv(p2 h)), pref = h   (autoent.cpp:52-56)
+};
+
+// off[0..3] index density arrays in one device buffer D (doubles).
+struct RedItem {
+  uint64_t off[4];
+  uint64_t n;
+  uint32_t op;
+  uint32_t pad;
+  double pref;   // TE prefactor / autoent bandwidth
+  double scale;  // 1 for ln, 1/ln 2 for bits
+};
+
+void launch_reduce(hipStream_t s, const RedItem* items, uint32_t n_items, const double* D,
+                   double* partial, uint32_t blocks_per_item, double* out);
+
+}  // namespace ent

@@ -1,0 +1,169 @@
+/* entropical_kde.h -- C ABI of the MI355X (gfx950) Epanechnikov KDE hot path.
+ *
+ * Drop-in boundary for lettis/entropical's density backend.  Each entry
+ * point names the reference interface it replaces (file:line in the
+ * lettis/entropical source tree).  All functions return an ENT_* status;
+ * on failure ent_last_error() holds a message (per calling thread).
+ *
+ * Data layout: coordinates are column-major float32, coords[col*n_rows+row],
+ * as Tools::IO::read_coords(..., 'C') produces them (tools.hxx:226-228).
+ * Densities are returned as double, divided by the ORIGINAL n_rows, with
+ * n_rows - max(tau) entries (densities_cuda.cu:95-103).
+ *
+ * Threading: one context per thread/device; a context is not re-entrant.
+ */
+#ifndef ENTROPICAL_KDE_H
+#define ENTROPICAL_KDE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ENT_OK 0
+#define ENT_ERR_ARG 1         /* bad dims, |h| != d, |tau| != d, tau >= n_rows, column out of range, h <= 0 */
+#define ENT_ERR_NO_DEVICE 2   /* no HIP device (the reference exits: density_1d.cuh:190-193) */
+#define ENT_ERR_HIP 3         /* HIP runtime error (replaces check_error, tools.cuh:33-42) */
+#define ENT_ERR_OOM 4
+#define ENT_ERR_NONFINITE 5   /* NaN/Inf coordinate */
+#define ENT_ERR_UNSUPPORTED 6 /* configuration outside the engine's limits */
+
+#define ENT_METHOD_CELL 0     /* cell-list candidates (default) */
+#define ENT_METHOD_BRUTE 1    /* all pairs, as the reference's CUDA backend does */
+
+typedef struct ent_ctx ent_ctx;
+
+/* One density estimate in a batch.  Point k (0 <= k < n), dim e:
+ * X[off[e] + k].  Densities go to dens[out_off + k], divided by n_rows. */
+typedef struct ent_job {
+  uint32_t n_dim;
+  uint32_t n;
+  uint32_t n_rows;
+  uint32_t reserved;
+  uint64_t off[3];
+  float h[3];
+  float reserved2;
+  uint64_t out_off;
+} ent_job;
+
+/* Library version (no reference counterpart). */
+int ent_version(void);
+
+/* Message of the last failing call on this thread.  Replaces the reference's
+ * std::cerr + exit(EXIT_FAILURE) error path (densities_common.hxx:16-33,
+ * tools.cuh:33-42). */
+const char* ent_last_error(void);
+
+/* Number of visible HIP devices (replaces cudaGetDeviceCount,
+ * density_1d.cuh:188-193). */
+int ent_device_count(int* n);
+
+/* Context = device + stream + grow-only workspace.  Replaces the per-call
+ * cudaSetDevice/cudaMalloc/cudaFree of density_Nd_per_gpu
+ * (density_1d.cuh:105-111,163-164). */
+int ent_ctx_create(int device, ent_ctx** out);
+int ent_ctx_destroy(ent_ctx* ctx);
+
+/* Engine tuning: queries per lane (1,2,4,8), dim-0 boxes per bandwidth,
+ * row-dim boxes per bandwidth, tile extent bound in bandwidths.  Replaces
+ * the reference's compile-time BSIZE (densities_cuda.cu:14). */
+int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h);
+
+/* Statistics of the last density run: persistent-kernel time and whole
+ * pipeline time (HIP events, ms), executed pair evaluations, tiles. */
+int ent_ctx_last_stats(ent_ctx* ctx, double* density_ms, double* total_ms, uint64_t* pairs,
+                       uint64_t* tiles);
+
+/* Executed pair evaluations of the last density run, split by job dimension
+ * (out[d-1] for d = 1, 2, 3). */
+int ent_ctx_last_pairs(ent_ctx* ctx, uint64_t* out3);
+
+/* Device memory helpers on the context's device and stream. */
+int ent_dev_alloc(ent_ctx* ctx, size_t bytes, void** ptr);
+int ent_dev_free(ent_ctx* ctx, void* ptr);
+int ent_h2d(ent_ctx* ctx, void* dst, const void* src, size_t bytes);
+int ent_d2h(ent_ctx* ctx, void* dst, const void* src, size_t bytes);
+int ent_ctx_sync(ent_ctx* ctx);
+
+/* prob_dens_coord_prep<float> (densities_common.hpp:11-18,
+ * densities_common.hxx:7-50): lagged gather of the selected columns into a
+ * (n_rows - tau_max) x n_dim array, row- or column-major.  Host only. */
+int ent_coord_prep(const float* coords, size_t n_rows, const uint32_t* i_cols, const float* h,
+                   const uint32_t* tau, uint32_t n_dim, int row_major, float* out, size_t* n_sel);
+
+/* combined_densities(coords, n_rows, i_cols, h, tau) -> vector<double>
+ * (densities_cuda.hpp:10-24, densities_cuda.cu:16-105).  Host pointers;
+ * n_cols bounds i_cols; tau may be NULL (all zero, densities_cuda.cu:17-44).
+ * out receives n_rows - max(tau) doubles. */
+int ent_combined_densities(ent_ctx* ctx, const float* coords, size_t n_rows, size_t n_cols,
+                           const uint32_t* i_cols, const float* h, const uint32_t* tau,
+                           uint32_t n_dim, int method, double* out);
+
+/* Same, with coords and out in device memory of the context's device. */
+int ent_combined_densities_dev(ent_ctx* ctx, const float* coords_dev, size_t n_rows, size_t n_cols,
+                               const uint32_t* i_cols, const float* h, const uint32_t* tau,
+                               uint32_t n_dim, int method, double* out_dev);
+
+/* Many density estimates in one engine run (no reference counterpart: the
+ * reference makes one combined_densities call per density, mi.cpp:37-52).
+ * X_dev and dens_dev are device pointers; jobs is a host array. */
+int ent_batch_densities_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs,
+                            int method, double* dens_dev);
+
+/* epa_convolution(sorted_coords, h) (densities_cuda.hpp:29-31) with the
+ * OpenMP semantics (densities_omp.cpp:235-277):
+ * (1/N^2) sum_i sum_j [d <= 2] (4.5 - 2.25 d), d = |x_i - x_j| / h.
+ * Host input (any order), result in *out. */
+int ent_epa_convolution(ent_ctx* ctx, const float* x, size_t n, float h, float* out);
+
+/* Transs::BoxedSearch::Boxes binning of one column (boxedsearch.cpp:15-34):
+ * box = floor((x - min) / box_size), n_boxes = ceil((max - min) / box_size),
+ * with max initialised to the lowest float, n_boxes >= 1 and x == max
+ * clamped to n_boxes - 1.  Computed on the GPU; host in/out. */
+int ent_bin(ent_ctx* ctx, const float* x, size_t n, float box_size, int32_t* box,
+            uint32_t* n_boxes, float* mn, float* mx);
+
+/* Cell list of one density job (no reference counterpart; the multi-dim
+ * generalisation of Boxes): stable sort of the job's points by cell key
+ * ((b2*nb1 + b1)*nb0 + b0) with box sizes h_e/m (m = tuning m0 for dim 0,
+ * mr for the others).  Host in/out: keys_sorted and perm have
+ * n_rows - max(tau) entries; nb, mn, bs have n_dim entries. */
+int ent_cell_sort(ent_ctx* ctx, const float* coords, size_t n_rows, size_t n_cols,
+                  const uint32_t* i_cols, const float* h, const uint32_t* tau, uint32_t n_dim,
+                  uint64_t* keys_sorted, uint32_t* perm, uint32_t* nb, float* mn, float* bs);
+
+/* Mi::main (mi.cpp:14-69) as one batch: n_cols 1-D marginals (once each;
+ * the reference recomputes p_j per pair, mi.cpp:43-46) and n_cols*(n_cols-1)/2
+ * 2-D joints in one engine run, then
+ * MI[i][j] = sum_k p_ij log(p_ij / p_i / p_j) reduced on the device in FP64
+ * (the reference accumulates in FP32 with logf, mi.cpp:53-56).  Mirrored,
+ * diagonal 0.  X_dev: device column-major coordinates; h: host, n_cols
+ * bandwidths; bits != 0 selects log2 (select_log, tools.cpp:18-37); mi: host
+ * n_cols*n_cols doubles. */
+int ent_mi_matrix_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
+                      int bits, int method, double* mi);
+
+/* Transs::main (transs.cpp:16-125) for several lags in one batch.
+ * te[t*n_cols*n_cols + i*n_cols + j] has the layout of the reference's
+ * transs matrix for lag taus[t] (TE(j->i) at [i][j], transs.cpp:84,102),
+ * including its prefactors (h_i h_i h_j and h_i h_j h_i) and its rejection of
+ * non-finite terms and p3 > 1.  FP64 reduction on the device. */
+int ent_te_matrix_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
+                      const uint32_t* taus, size_t n_taus, int bits, int method, double* te);
+
+/* Resubstitution entropy H = -(1/n) sum_k ln p(x_k) of every job in one
+ * batch (no reference counterpart; the closest reference output is
+ * Negs::main, negs.cpp:37-47).  H: host, n_jobs doubles. */
+int ent_batch_entropy_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs,
+                          int method, double* H);
+
+/* Wall time (ms) of the last driver call on this context. */
+int ent_ctx_last_driver_ms(ent_ctx* ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ENTROPICAL_KDE_H */

@@ -1,0 +1,202 @@
+"""GPU parity: the HIP density path (through the C ABI) against the FP64
+oracle.  Tolerance: densities within 1e-5 relative of FP64 (FP32 compensated
+accumulation on the device, FP64 normalisation)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+def data(seed, n_cols, n, scale=1.0, kind="normal"):
+    r = np.random.default_rng(seed)
+    if kind == "normal":
+        x = r.standard_normal((n_cols, n))
+    elif kind == "t4":
+        x = r.standard_t(4, size=(n_cols, n))
+    elif kind == "uniform":
+        x = r.uniform(-1, 1, size=(n_cols, n))
+    else:
+        raise ValueError(kind)
+    return (scale * x).astype(np.float32)
+
+
+@pytest.mark.parametrize("method", ["cell", "brute"])
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_density_matches_fp64(ctx, orc, method, d):
+    from entropical_amd import combined_densities
+
+    n = 3000
+    x = data(100 + d, 3, n)
+    cols = [0, 1, 2][:d]
+    h = [0.2, 0.3, 0.45][:d]
+    got = combined_densities(x, n, cols, h, method=method, ctx=ctx)
+    ref = orc.density_fp64(x, n, cols, h)
+    assert got.shape == ref.shape
+    assert rel_err(got, ref) < RTOL
+
+
+@pytest.mark.parametrize("method", ["cell", "brute"])
+def test_lagged_and_repeated_columns(ctx, orc, method):
+    from entropical_amd import combined_densities
+
+    n = 2500
+    x = data(7, 2, n)
+    for cols, tau in (([0, 0], [0, 5]), ([0, 0, 1], [0, 3, 0]), ([0, 1, 1], [0, 0, 10]), ([1], [4])):
+        h = [0.3, 0.25, 0.4][: len(cols)]
+        got = combined_densities(x, n, cols, h, tau, method=method, ctx=ctx)
+        ref = orc.density_fp64(x, n, cols, h, tau)
+        assert got.shape == (n - max(tau),)
+        assert rel_err(got, ref) < RTOL, (cols, tau)
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 257, 4099])
+def test_small_and_ragged_sizes(ctx, orc, n):
+    from entropical_amd import combined_densities
+
+    x = data(n, 3, n)
+    for d in (1, 2, 3):
+        cols = list(range(d))
+        h = [0.5] * d
+        ref = orc.density_fp64(x, n, cols, h)
+        for method in ("cell", "brute"):
+            got = combined_densities(x, n, cols, h, method=method, ctx=ctx)
+            assert rel_err(got, ref) < RTOL, (d, method)
+
+
+def test_heavy_tails_and_ties(ctx, orc):
+    from entropical_amd import combined_densities
+
+    n = 6000
+    x = data(11, 3, n, kind="t4", scale=3.0)
+    x[1, ::7] = 0.25  # many exact ties in one dim
+    x[2, 100:200] = x[2, 100]  # a block of identical values
+    for cols, h in (([0], [0.05]), ([0, 1], [0.1, 0.07]), ([0, 1, 2], [0.3, 0.2, 0.25])):
+        got = combined_densities(x, n, cols, h, ctx=ctx)
+        ref = orc.density_fp64_window(x, n, cols, h)
+        assert rel_err(got, ref) < RTOL, cols
+
+
+def test_all_points_identical(ctx):
+    from entropical_amd import combined_densities
+
+    n = 1000
+    x = np.full((2, n), 1.5, np.float32)
+    got = combined_densities(x, n, [0, 1], [0.2, 0.4], ctx=ctx)
+    h1, h2 = float(np.float32(0.2)), float(np.float32(0.4))
+    np.testing.assert_allclose(got, (0.75 / h1) * (0.75 / h2), rtol=1e-6)
+
+
+@pytest.mark.parametrize("tuning", [(1, 1, 1, 1), (2, 2, 1, 4), (8, 8, 2, 1), (4, 4, 3, 2)])
+def test_tunings_agree(ctx, orc, tuning):
+    from entropical_amd import combined_densities
+
+    n = 5000
+    x = data(21, 3, n)
+    ctx.set_tuning(*tuning)
+    try:
+        for d in (1, 2, 3):
+            cols = list(range(d))
+            h = [0.15, 0.2, 0.3][:d]
+            got = combined_densities(x, n, cols, h, ctx=ctx)
+            ref = orc.density_fp64_window(x, n, cols, h)
+            assert rel_err(got, ref) < RTOL, (tuning, d)
+    finally:
+        ctx.set_tuning()
+
+
+def test_deterministic(ctx):
+    from entropical_amd import combined_densities
+
+    n = 20000
+    x = data(31, 2, n)
+    a = combined_densities(x, n, [0, 1], [0.1, 0.1], ctx=ctx)
+    b = combined_densities(x, n, [0, 1], [0.1, 0.1], ctx=ctx)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_nonfinite_is_rejected(ctx):
+    from entropical_amd import EntropicalError, combined_densities
+
+    x = data(1, 1, 100)
+    x[0, 17] = np.nan
+    with pytest.raises(EntropicalError, match="non-finite"):
+        combined_densities(x, 100, [0], [0.3], ctx=ctx)
+
+
+# ------------------------------------------------------------- binning
+def test_bin_bit_exact_vs_oracle(ctx, orc):
+    from entropical_amd import bin_column
+
+    r = np.random.default_rng(5)
+    for trial in range(30):
+        n = int(r.integers(1, 50000))
+        x = (r.standard_normal(n) * r.uniform(0.01, 50) + r.uniform(-100, 100)).astype(np.float32)
+        bs = np.float32(r.uniform(1e-3, 2.0))
+        g_box, g_nb, g_mn, g_mx = bin_column(x, bs, ctx=ctx)
+        o_box, o_nb, o_mn, o_mx = orc.bin_column(x, bs)
+        assert (g_nb, g_mn, g_mx) == (o_nb, o_mn, o_mx)
+        np.testing.assert_array_equal(g_box, o_box)
+
+
+def test_bin_edge_cases(ctx, orc):
+    from entropical_amd import bin_column
+
+    cases = [
+        (np.array([0.0, 0.1, 0.2, 0.3, 0.95, 1.0], np.float32), 0.25),
+        (np.full(5, -2.5, np.float32), 0.1),
+        (np.array([-3.0, -2.0, -1.5], np.float32), 0.5),
+        (np.arange(1000, dtype=np.float32) * np.float32(0.1), np.float32(0.1)),
+    ]
+    for x, bs in cases:
+        g = bin_column(x, bs, ctx=ctx)
+        o = orc.bin_column(x, bs)
+        assert g[1:] == o[1:]
+        np.testing.assert_array_equal(g[0], o[0])
+
+
+def test_cell_sort_bit_exact(ctx, orc):
+    """GPU stable cell sort == CPU stable counting sort by the same keys."""
+    from entropical_amd import cell_sort
+
+    r = np.random.default_rng(9)
+    n = 30000
+    x = r.standard_normal((3, n)).astype(np.float32)
+    for cols, h, tau in (([0], [0.1], None), ([0, 1], [0.2, 0.3], None), ([0, 1, 2], [0.3, 0.2, 0.4], [0, 2, 1])):
+        keys, perm, nb, mn, bs = cell_sort(x, n, cols, h, tau, ctx=ctx)
+        d = len(cols)
+        tau = tau or [0] * d
+        ns = n - max(tau)
+        # CPU: boxes per dim with the oracle's binning, same box sizes
+        boxes = []
+        for e in range(d):
+            col = x[cols[e], tau[e]: tau[e] + ns]
+            b, onb, omn, _ = orc.bin_column(col, bs[e])
+            assert onb == nb[e] and omn == mn[e]
+            boxes.append(b.astype(np.uint64))
+        key = boxes[0].copy()
+        if d >= 2:
+            row = boxes[1].copy()
+            if d == 3:
+                row += boxes[2] * np.uint64(nb[1])
+            key += row * np.uint64(nb[0])
+        order = np.argsort(key, kind="stable")
+        np.testing.assert_array_equal(perm, order.astype(np.uint32))
+        np.testing.assert_array_equal(keys, key[order])
+
+
+# ---------------------------------------------------------- epa convolution
+def test_epa_convolution_matches_fp64(ctx, orc):
+    from entropical_amd import epa_convolution
+
+    r = np.random.default_rng(3)
+    for n, h in ((1, 0.3), (2, 0.3), (5000, 0.05), (20000, 0.2)):
+        x = np.sort(r.standard_normal(n).astype(np.float32))
+        got = epa_convolution(x, h, ctx=ctx)
+        ref = orc.epa_conv_fp64(x, h)
+        assert got == pytest.approx(ref, rel=RTOL)
