@@ -82,6 +82,18 @@ def build_library(jobs: int = 8, verbose: bool = True) -> Path:
     return LIB
 
 
+def build_dropin_check(verbose: bool = True) -> Path:
+    """C++ program linking the drop-in header against the library (used by
+    tests/test_gpu_dropin.py)."""
+    src = ROOT / "tests" / "cpp" / "dropin_check.cpp"
+    exe = src.with_suffix("")
+    _run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(src), f"-L{PKG}", "-lentropical_kde",
+          f"-Wl,-rpath,$ORIGIN/../../entropical_amd", "-o", str(exe)])
+    if verbose:
+        print(f"built {exe}")
+    return exe
+
+
 def build_oracle(verbose: bool = True, with_ref: bool = True) -> None:
     od = ROOT / "oracle"
     _run(["make", "-C", str(od), "-j4", "all"])
@@ -97,6 +109,7 @@ def main(argv=None) -> int:
     ap.add_argument("-j", type=int, default=8)
     a = ap.parse_args(argv)
     build_library(a.j)
+    build_dropin_check()
     if a.oracle:
         build_oracle()
     return 0
