@@ -166,6 +166,12 @@ int ent_ctx_last_stats(ent_ctx* ctx, double* density_ms, double* total_ms, uint6
   return ENT_OK;
 }
 
+int ent_ctx_set_graph(ent_ctx* ctx, int enable) {
+  CAPI_TRY(check_ctx(ctx));
+  ctx->eng->use_graph = enable != 0;
+  return ENT_OK;
+}
+
 int ent_ctx_last_pairs(ent_ctx* ctx, uint64_t* out3) {
   CAPI_TRY(check_ctx(ctx));
   if (!out3) {

@@ -53,6 +53,8 @@ Engine::~Engine() {
   DevBuf* bufs[] = {&d_jobs,  &d_chunks, &d_chunk_first, &d_mm,     &d_flags,        &d_keys[0],
                     &d_keys[1], &d_idx[0], &d_idx[1],     &d_hist,   &d_scan_scratch, &d_scoord,
                     &d_tile_cnt, &d_tiles, &d_counter};
+  for (GraphCache* c : {&pre_graphs_, &dens_graphs_})
+    for (auto& e : c->entries) hipGraphExecDestroy(e.second);
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   for (auto& e : ev_)
@@ -70,6 +72,7 @@ int Engine::ensure(DevBuf& b, size_t bytes) {
   size_t want = std::max<size_t>(bytes + bytes / 8, 4096);
   ENT_HIP(hipMalloc(&b.p, want));
   b.cap = want;
+  ++generation_;
   return kOk;
 }
 
@@ -251,30 +254,100 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   uint32_t* hist = (uint32_t*)d_hist.p;
   uint32_t* scratch = (uint32_t*)d_scan_scratch.p;
 
-  launch_cell_keys(stream_, dj, dch, NC, X, keys[0], idx[0]);
-  int cur = 0;
-  for (int pass = 0; pass < npass; ++pass) {
-    int shift = pass * kRadixBits;
-    launch_radix_hist(stream_, dj, dch, dcf, NC, keys[cur], shift, hist);
-    launch_exclusive_scan(stream_, hist, hist, HM, scratch);
-    launch_radix_scatter(stream_, dj, dch, dcf, NC, keys[cur], idx[cur], keys[cur ^ 1], idx[cur ^ 1],
-                         shift, hist);
-    cur ^= 1;
-  }
-  sorted_buf = cur;
+  sorted_buf = npass & 1;
+  const uint32_t tq = (uint32_t)(kWave * tuning.q);
+  uint32_t* tcnt = (uint32_t*)d_tile_cnt.p;
+  // keys -> radix passes [-> gather -> tiles]; a fixed launch sequence for a
+  // given plan shape, replayed from a captured graph when the shape repeats
+  auto issue_pre = [&](bool with_tiles) {
+    launch_cell_keys(stream_, dj, dch, NC, X, keys[0], idx[0]);
+    int cur = 0;
+    for (int pass = 0; pass < npass; ++pass) {
+      int shift = pass * kRadixBits;
+      launch_radix_hist(stream_, dj, dch, dcf, NC, keys[cur], shift, hist);
+      launch_exclusive_scan(stream_, hist, hist, HM, scratch);
+      launch_radix_scatter(stream_, dj, dch, dcf, NC, keys[cur], idx[cur], keys[cur ^ 1], idx[cur ^ 1],
+                           shift, hist);
+      cur ^= 1;
+    }
+    if (!with_tiles) return;
+    launch_gather_sorted(stream_, dj, dch, NC, X, idx[cur], (float*)d_scoord.p);
+    launch_tile_count(stream_, dj, dch, NC, keys[cur], tq, tcnt);
+    launch_exclusive_scan(stream_, tcnt, tcnt, NC, scratch);
+    launch_tile_write(stream_, dj, dch, NC, keys[cur], tq, tcnt, (DTile*)d_tiles.p);
+    launch_tile_ends(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
+  };
   if (stop == kStopAfterSort) {
+    issue_pre(false);
     ENT_HIP(hipGetLastError());
     return kOk;
   }
-  launch_gather_sorted(stream_, dj, dch, NC, X, idx[cur], (float*)d_scoord.p);
-  const uint32_t tq = (uint32_t)(kWave * tuning.q);
-  uint32_t* tcnt = (uint32_t*)d_tile_cnt.p;
-  launch_tile_count(stream_, dj, dch, NC, keys[cur], tq, tcnt);
-  launch_exclusive_scan(stream_, tcnt, tcnt, NC, scratch);
-  launch_tile_write(stream_, dj, dch, NC, keys[cur], tq, tcnt, (DTile*)d_tiles.p);
-  launch_tile_ends(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
-  ENT_HIP(hipGetLastError());
+  PlanKey key = plan_key(X, dens, J, NC, P, npass, false);
+  ENT_TRY(run_captured(pre_graphs_, key, [&] { issue_pre(true); }));
   return launch_density_common(false, X, dens, P);
+}
+
+PlanKey Engine::plan_key(const float* X, const double* dens, uint32_t J, uint32_t NC, uint64_t P, int npass,
+                         bool raw) const {
+  PlanKey k;
+  k.X = X;
+  k.dens = dens;
+  k.J = J;
+  k.NC = NC;
+  k.P = P;
+  k.npass = npass;
+  k.raw = raw ? 1 : 0;
+  k.q = tuning.q;
+  k.m0 = tuning.m0;
+  k.mr = tuning.mr;
+  k.gh = tuning.group_h;
+  k.gen = generation_;
+  return k;
+}
+
+template <class F>
+int Engine::run_captured(GraphCache& cache, const PlanKey& key, F&& issue) {
+  if (!use_graph) {
+    issue();
+    ENT_HIP(hipGetLastError());
+    return kOk;
+  }
+  for (size_t i = 0; i < cache.entries.size(); ++i) {
+    if (cache.entries[i].first == key) {
+      auto hit = cache.entries[i];
+      cache.entries.erase(cache.entries.begin() + i);
+      cache.entries.push_back(hit);
+      ENT_HIP(hipGraphLaunch(hit.second, stream_));
+      ++last.graph_replays;
+      return kOk;
+    }
+  }
+  if (cache.entries.size() >= GraphCache::kMax) {
+    // the stream may still run the oldest graph; wait before destroying it
+    ENT_HIP(hipStreamSynchronize(stream_));
+    hipGraphExecDestroy(cache.entries.front().second);
+    cache.entries.erase(cache.entries.begin());
+  }
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t g = nullptr;
+  ENT_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  issue();
+  hipError_t le = hipGetLastError();
+  hipError_t ce = hipStreamEndCapture(stream_, &g);
+  if (le != hipSuccess || ce != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    set_error(std::string("graph capture failed: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+    return kErrHip;
+  }
+  hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ie != hipSuccess) {
+    set_error(std::string("graph instantiation failed: ") + hipGetErrorString(ie));
+    return kErrHip;
+  }
+  cache.entries.emplace_back(key, exec);
+  ENT_HIP(hipGraphLaunch(exec, stream_));
+  return kOk;
 }
 
 int Engine::run_brute(const float* X, const std::vector<JobSpec>& jobs, double* dens) {
@@ -321,18 +394,30 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
   const uint32_t NC = (uint32_t)h_chunks.size();
   const uint32_t* n_tiles = raw ? (const uint32_t*)d_tile_cnt.p : (const uint32_t*)d_tile_cnt.p + NC;
   uint32_t* counter = (uint32_t*)d_counter.p;
-  launch_fill_u32(stream_, counter, 0u, 16);
+  int rc = 0;
+  // the work counter and pair counters are cleared by a fill kernel (not a
+  // memset node) inside the same graph as the persistent launch
+  auto issue = [&] {
+    launch_fill_u32(stream_, counter, 0u, 16);
+    rc = launch_density(stream_, tuning.q, raw, grid, (const DJob*)d_jobs.p, (const DTile*)d_tiles.p, n_tiles,
+                        counter, X, raw ? nullptr : (const float*)d_scoord.p,
+                        raw ? nullptr : (const uint64_t*)d_keys[sorted_buf].p,
+                        raw ? nullptr : (const uint32_t*)d_idx[sorted_buf].p, dens,
+                        (unsigned long long*)(counter + 2));
+  };
   ENT_HIP(hipEventRecord(ev_[1], stream_));
-  int rc = launch_density(stream_, tuning.q, raw, grid, (const DJob*)d_jobs.p, (const DTile*)d_tiles.p,
-                          n_tiles, counter, X, raw ? nullptr : (const float*)d_scoord.p,
-                          raw ? nullptr : (const uint64_t*)d_keys[sorted_buf].p,
-                          raw ? nullptr : (const uint32_t*)d_idx[sorted_buf].p, dens,
-                          (unsigned long long*)(counter + 2));
+  if (raw) {
+    issue();
+    ENT_HIP(hipGetLastError());
+  } else {
+    PlanKey key = plan_key(X, dens, (uint32_t)h_jobs.size(), NC, total_points, sorted_buf, false);
+    key.grid = (int)grid;
+    ENT_TRY(run_captured(dens_graphs_, key, issue));
+  }
   if (rc != 0) {
     set_error("density launch: unsupported configuration");
     return kErrUnsupported;
   }
-  ENT_HIP(hipGetLastError());
   ENT_HIP(hipEventRecord(ev_[2], stream_));
   // stats readback
   ENT_TRY(ensure_host(64));

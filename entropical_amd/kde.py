@@ -68,6 +68,9 @@ class Context:
     def set_tuning(self, q: int = 4, m0: int = 4, mr: int = 2, group_h: int = 2) -> None:
         check(load().ent_ctx_set_tuning(self.handle, q, m0, mr, group_h))
 
+    def set_graph(self, enable: bool) -> None:
+        check(load().ent_ctx_set_graph(self.handle, 1 if enable else 0))
+
     def last_stats(self) -> dict:
         dm, tm = C.c_double(), C.c_double()
         pairs, tiles = C.c_uint64(), C.c_uint64()

@@ -76,6 +76,12 @@ int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h);
 int ent_ctx_last_stats(ent_ctx* ctx, double* density_ms, double* total_ms, uint64_t* pairs,
                        uint64_t* tiles);
 
+/* Capture the post-planning launch sequence (cell keys, radix passes, gather,
+ * tiles) and the persistent density launch in hipGraphs that are replayed
+ * while the plan shape repeats (default on); 0 launches eagerly.  No
+ * reference counterpart. */
+int ent_ctx_set_graph(ent_ctx* ctx, int enable);
+
 /* Executed pair evaluations of the last density run, split by job dimension
  * (out[d-1] for d = 1, 2, 3). */
 int ent_ctx_last_pairs(ent_ctx* ctx, uint64_t* out3);

@@ -40,6 +40,31 @@ def test_mi_matrix(ctx, orc, method, bits):
             assert abs(got[i, j] - ref) <= RTOL * abs(ref), (i, j, got[i, j], ref)
 
 
+def test_graph_replays_match_eager(ctx):
+    """The captured preprocessing + density graphs, replayed several times with
+    eager work in between, give bit-identical results to eager launches."""
+    from entropical_amd import DeviceArray, combined_densities, mi_matrix
+
+    c, n = 6, 6000
+    x = coupled(5, c, n)
+    h = [0.3] * c
+    dx = DeviceArray(ctx, x)
+    try:
+        ctx.set_graph(False)
+        eager = mi_matrix(ctx, dx, h)
+        ctx.set_graph(True)
+        for rep in range(4):
+            got = mi_matrix(ctx, dx, h)
+            np.testing.assert_array_equal(got, eager)
+            if rep == 1:
+                assert ctx.last_stats()["density_ms"] > 0
+            # unrelated eager work between replays (different plan shape)
+            combined_densities(x, n, [0, 1, 2], [0.4] * 3, ctx=ctx)
+    finally:
+        ctx.set_graph(True)
+        dx.free()
+
+
 def test_te_matrix(ctx, orc):
     from entropical_amd import DeviceArray, te_matrix
 
