@@ -85,6 +85,14 @@ SIGNATURES = {
     "ent_te_matrix_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P, C.c_size_t, C.c_int, C.c_int, P]),
     "ent_batch_entropy_dev": (C.c_int, [P, P, C.POINTER(ent_job), C.c_size_t, C.c_int, P]),
     "ent_ctx_last_driver_ms": (C.c_int, [P, F64P]),
+    "ent_mlp_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]),
+    "ent_mlp_destroy": (C.c_int, [P]),
+    "ent_mlp_num_params": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+    "ent_mlp_get_params": (C.c_int, [P, P, P]),
+    "ent_mlp_set_params": (C.c_int, [P, P, P]),
+    "ent_mlp_predict": (C.c_int, [P, P, P, C.c_size_t, P]),
+    "ent_mlp_gradient": (C.c_int, [P, P, P, P, C.c_size_t, P, F32P]),
+    "ent_mlp_train": (C.c_int, [P, P, P, P, C.c_size_t, P, C.c_int, C.c_int, C.c_float, P]),
 }
 
 _lib = None
@@ -103,7 +111,9 @@ def load() -> C.CDLL:
         )
     lib = C.CDLL(str(path))
     for name, (res, args) in SIGNATURES.items():
-        fn = getattr(lib, name)
+        fn = getattr(lib, name, None)
+        if fn is None:  # an older library build; the symbol test covers the shipped one
+            continue
         fn.restype = res
         fn.argtypes = args
     _lib = lib

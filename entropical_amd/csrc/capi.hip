@@ -143,9 +143,8 @@ int ent_ctx_destroy(ent_ctx* ctx) {
 
 int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h) {
   CAPI_TRY(check_ctx(ctx));
-  if (!(q == 1 || q == 2 || q == 4 || q == 8) || m0 < 1 || m0 > 64 || mr < 1 || mr > 8 || group_h < 1 ||
-      group_h > 64) {
-    set_error("invalid tuning (q in {1,2,4,8}, 1<=m0<=64, 1<=mr<=8, 1<=group_h<=64)");
+  if (!(q == 1 || q == 2 || q == 4) || m0 < 1 || m0 > 64 || mr < 1 || mr > 8 || group_h < 1 || group_h > 64) {
+    set_error("invalid tuning (q in {1,2,4}, 1<=m0<=64, 1<=mr<=8, 1<=group_h<=64)");
     return ENT_ERR_ARG;
   }
   ctx->eng->tuning.q = q;
@@ -172,13 +171,14 @@ int ent_ctx_set_graph(ent_ctx* ctx, int enable) {
   return ENT_OK;
 }
 
-int ent_ctx_last_pairs(ent_ctx* ctx, uint64_t* out3) {
+int ent_ctx_last_pairs(ent_ctx* ctx, uint64_t* out3) {  // out3: 4 entries
   CAPI_TRY(check_ctx(ctx));
   if (!out3) {
     set_error("null argument");
     return ENT_ERR_ARG;
   }
   for (int d = 0; d < 3; ++d) out3[d] = ctx->eng->last.pairs_by_d[d];
+  out3[3] = ctx->eng->last.lane_pairs;
   return ENT_OK;
 }
 
@@ -459,11 +459,19 @@ int ent_cell_sort(ent_ctx* ctx, const float* coords, size_t n_rows, size_t n_col
   std::vector<ent::JobSpec> jobs(1, js);
   CAPI_TRY(E.densities((const float*)ctx->upload.p, jobs, nullptr, ent::kMethodCell, ent::kStopAfterSort));
   const size_t ns = js.n;
-  CAPI_HIP(hipMemcpyAsync(keys_sorted, E.d_keys[E.sorted_buf].p, sizeof(uint64_t) * ns,
-                          hipMemcpyDeviceToHost, E.stream()));
+  std::vector<uint32_t> narrow;
+  if (E.keys_wide) {
+    CAPI_HIP(hipMemcpyAsync(keys_sorted, E.d_keys[E.sorted_buf].p, sizeof(uint64_t) * ns,
+                            hipMemcpyDeviceToHost, E.stream()));
+  } else {
+    narrow.resize(ns);
+    CAPI_HIP(hipMemcpyAsync(narrow.data(), E.d_keys[E.sorted_buf].p, sizeof(uint32_t) * ns,
+                            hipMemcpyDeviceToHost, E.stream()));
+  }
   CAPI_HIP(hipMemcpyAsync(perm, E.d_idx[E.sorted_buf].p, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost,
                           E.stream()));
   CAPI_HIP(hipStreamSynchronize(E.stream()));
+  for (size_t i = 0; i < narrow.size(); ++i) keys_sorted[i] = narrow[i];
   const ent::DJob& D = E.h_jobs[0];
   for (uint32_t e = 0; e < n_dim; ++e) {
     if (nb) nb[e] = D.nb[e];

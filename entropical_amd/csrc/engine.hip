@@ -52,7 +52,7 @@ Engine::~Engine() {
   if (stream_) hipStreamSynchronize(stream_);
   DevBuf* bufs[] = {&d_jobs,  &d_chunks, &d_chunk_first, &d_mm,     &d_flags,        &d_keys[0],
                     &d_keys[1], &d_idx[0], &d_idx[1],     &d_hist,   &d_scan_scratch, &d_scoord,
-                    &d_tile_cnt, &d_tiles, &d_counter};
+                    &d_tile_cnt, &d_tiles, &d_counter, &d_cellstart};
   for (GraphCache* c : {&pre_graphs_, &dens_graphs_})
     for (auto& e : c->entries) hipGraphExecDestroy(e.second);
   for (DevBuf* b : bufs)
@@ -199,6 +199,7 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   }
   // host planning: boxes per dim (boxedsearch.cpp:21-27 with the total fixes)
   uint64_t max_key = 0;
+  uint64_t table_total = 0;
   for (uint32_t j = 0; j < J; ++j) {
     DJob& D = h_jobs[j];
     uint64_t cells = 1;
@@ -225,12 +226,21 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
       }
     }
     D.group = (uint32_t)std::max(1, tuning.group_h * tuning.m0);
+    D.cells = cells;
+    if (cells + 1 <= kTableCellsPerPoint * (uint64_t)D.n + 4096) {
+      D.cbase = table_total;
+      table_total += cells + 1;
+    } else {
+      D.cbase = kNoTable;
+    }
     max_key = std::max<uint64_t>(max_key, cells - 1);
   }
   int bits = 0;
   while (bits < 64 && (max_key >> bits) != 0) ++bits;
   const int npass = (bits + kRadixBits - 1) / kRadixBits;
   last.radix_passes = (uint32_t)npass;
+  const bool wide = bits > 32;
+  keys_wide = wide;
   std::memcpy(hs_jobs, h_jobs.data(), sizeof(DJob) * J);
   ENT_HIP(hipMemcpyAsync(d_jobs.p, hs_jobs, sizeof(DJob) * J, hipMemcpyHostToDevice, stream_));
   if (stop == kStopAfterPlan) return kOk;
@@ -238,8 +248,9 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   // buffers sized by the plan
   const uint64_t HM = (uint64_t)kRadixBins * NC;
   const uint64_t scan_m = std::max<uint64_t>(HM, NC);
-  ENT_TRY(ensure(d_keys[0], sizeof(uint64_t) * P));
-  ENT_TRY(ensure(d_keys[1], sizeof(uint64_t) * P));
+  const size_t key_bytes = wide ? sizeof(uint64_t) : sizeof(uint32_t);
+  ENT_TRY(ensure(d_keys[0], key_bytes * P));
+  ENT_TRY(ensure(d_keys[1], key_bytes * P));
   ENT_TRY(ensure(d_idx[0], sizeof(uint32_t) * P));
   ENT_TRY(ensure(d_idx[1], sizeof(uint32_t) * P));
   ENT_TRY(ensure(d_hist, sizeof(uint32_t) * (HM + 1)));
@@ -248,8 +259,10 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   ENT_TRY(ensure(d_tile_cnt, sizeof(uint32_t) * (NC + 1)));
   ENT_TRY(ensure(d_tiles, sizeof(DTile) * P));
   ENT_TRY(ensure(d_counter, 64));
+  ENT_TRY(ensure(d_cellstart, sizeof(uint32_t) * std::max<uint64_t>(table_total, 1)));
+  last.table_cells = table_total;
 
-  uint64_t* keys[2] = {(uint64_t*)d_keys[0].p, (uint64_t*)d_keys[1].p};
+  void* keys[2] = {d_keys[0].p, d_keys[1].p};
   uint32_t* idx[2] = {(uint32_t*)d_idx[0].p, (uint32_t*)d_idx[1].p};
   uint32_t* hist = (uint32_t*)d_hist.p;
   uint32_t* scratch = (uint32_t*)d_scan_scratch.p;
@@ -260,21 +273,24 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   // keys -> radix passes [-> gather -> tiles]; a fixed launch sequence for a
   // given plan shape, replayed from a captured graph when the shape repeats
   auto issue_pre = [&](bool with_tiles) {
-    launch_cell_keys(stream_, dj, dch, NC, X, keys[0], idx[0]);
+    launch_cell_keys(stream_, wide, dj, dch, NC, X, keys[0], idx[0]);
     int cur = 0;
     for (int pass = 0; pass < npass; ++pass) {
       int shift = pass * kRadixBits;
-      launch_radix_hist(stream_, dj, dch, dcf, NC, keys[cur], shift, hist);
+      // the last pass also writes the sorted coordinates
+      float* sc = (with_tiles && pass == npass - 1) ? (float*)d_scoord.p : nullptr;
+      launch_radix_hist(stream_, wide, dj, dch, dcf, NC, keys[cur], shift, hist);
       launch_exclusive_scan(stream_, hist, hist, HM, scratch);
-      launch_radix_scatter(stream_, dj, dch, dcf, NC, keys[cur], idx[cur], keys[cur ^ 1], idx[cur ^ 1],
-                           shift, hist);
+      launch_radix_scatter(stream_, wide, dj, dch, dcf, NC, keys[cur], idx[cur], keys[cur ^ 1], idx[cur ^ 1],
+                           shift, hist, X, sc);
       cur ^= 1;
     }
     if (!with_tiles) return;
-    launch_gather_sorted(stream_, dj, dch, NC, X, idx[cur], (float*)d_scoord.p);
-    launch_tile_count(stream_, dj, dch, NC, keys[cur], tq, tcnt);
+    if (npass == 0) launch_gather_sorted(stream_, dj, dch, NC, X, idx[cur], (float*)d_scoord.p);
+    if (table_total) launch_cell_starts(stream_, wide, dj, dch, NC, keys[cur], (uint32_t*)d_cellstart.p);
+    launch_tile_count(stream_, wide, dj, dch, NC, keys[cur], tq, tcnt);
     launch_exclusive_scan(stream_, tcnt, tcnt, NC, scratch);
-    launch_tile_write(stream_, dj, dch, NC, keys[cur], tq, tcnt, (DTile*)d_tiles.p);
+    launch_tile_write(stream_, wide, dj, dch, NC, keys[cur], tq, tcnt, (DTile*)d_tiles.p);
     launch_tile_ends(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
   };
   if (stop == kStopAfterSort) {
@@ -386,7 +402,8 @@ int Engine::run_brute(const float* X, const std::vector<JobSpec>& jobs, double* 
 int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64_t max_tiles) {
   (void)max_tiles;
   int bpc = 0;
-  if (density_occupancy(tuning.q, raw, &bpc) != 0 || bpc <= 0) {
+  const bool wide = !raw && keys_wide;
+  if (density_occupancy(tuning.q, raw, wide, &bpc) != 0 || bpc <= 0) {
     set_error("unsupported queries-per-lane setting");
     return kErrUnsupported;
   }
@@ -399,9 +416,9 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
   // memset node) inside the same graph as the persistent launch
   auto issue = [&] {
     launch_fill_u32(stream_, counter, 0u, 16);
-    rc = launch_density(stream_, tuning.q, raw, grid, (const DJob*)d_jobs.p, (const DTile*)d_tiles.p, n_tiles,
-                        counter, X, raw ? nullptr : (const float*)d_scoord.p,
-                        raw ? nullptr : (const uint64_t*)d_keys[sorted_buf].p,
+    rc = launch_density(stream_, tuning.q, raw, wide, grid, (const DJob*)d_jobs.p, (const DTile*)d_tiles.p,
+                        n_tiles, counter, X, raw ? nullptr : (const float*)d_scoord.p,
+                        raw ? nullptr : d_keys[sorted_buf].p, raw ? nullptr : (const uint32_t*)d_cellstart.p,
                         raw ? nullptr : (const uint32_t*)d_idx[sorted_buf].p, dens,
                         (unsigned long long*)(counter + 2));
   };
@@ -412,6 +429,7 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
   } else {
     PlanKey key = plan_key(X, dens, (uint32_t)h_jobs.size(), NC, total_points, sorted_buf, false);
     key.grid = (int)grid;
+    key.raw = wide ? 2 : 0;  // key width selects the kernel variant
     ENT_TRY(run_captured(dens_graphs_, key, issue));
   }
   if (rc != 0) {
@@ -422,14 +440,15 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
   // stats readback
   ENT_TRY(ensure_host(64));
   uint32_t* hc = (uint32_t*)host_stage_;
-  ENT_HIP(hipMemcpyAsync(hc, counter, 32, hipMemcpyDeviceToHost, stream_));
-  ENT_HIP(hipMemcpyAsync(hc + 8, n_tiles, 4, hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipMemcpyAsync(hc, counter, 40, hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipMemcpyAsync(hc + 10, n_tiles, 4, hipMemcpyDeviceToHost, stream_));
   ENT_HIP(hipStreamSynchronize(stream_));
-  uint64_t pd[3];
-  std::memcpy(pd, hc + 2, 24);
+  uint64_t pd[4];
+  std::memcpy(pd, hc + 2, 32);
+  last.lane_pairs = pd[3];
   last.pairs = pd[0] + pd[1] + pd[2];
   for (int d = 0; d < 3; ++d) last.pairs_by_d[d] = pd[d];
-  last.tiles = hc[8];
+  last.tiles = hc[10];
   float ms = 0.f;
   ENT_HIP(hipEventElapsedTime(&ms, ev_[1], ev_[2]));
   last.density_ms = ms;

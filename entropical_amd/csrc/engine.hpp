@@ -64,8 +64,10 @@ struct RunStats {
   double plan_ms = 0.0;      // minmax + host planning round trip
   uint64_t pairs = 0;        // executed (query, candidate) evaluations
   uint64_t pairs_by_d[3] = {0, 0, 0};  // the same, per job dimension
+  uint64_t lane_pairs = 0;   // lane-slot evaluations issued (incl. idle lanes / padding)
   uint64_t tiles = 0;
   uint32_t radix_passes = 0;
+  uint64_t table_cells = 0;  // cell-start table entries of the last plan
 };
 
 class Engine {
@@ -122,13 +124,14 @@ class Engine {
  public:
   // workspace buffers
   DevBuf d_jobs, d_chunks, d_chunk_first, d_mm, d_flags, d_keys[2], d_idx[2], d_hist,
-      d_scan_scratch, d_scoord, d_tile_cnt, d_tiles, d_counter;
+      d_scan_scratch, d_scoord, d_tile_cnt, d_tiles, d_counter, d_cellstart;
   // host copies
   std::vector<DJob> h_jobs;
   std::vector<uint2> h_chunks;
   std::vector<uint32_t> h_chunk_first;
   uint64_t total_points = 0;
   int sorted_buf = 0;  // which of d_keys/d_idx holds the sorted result
+  bool keys_wide = false;  // cell keys stored as uint64 (else uint32)
 };
 
 }  // namespace ent

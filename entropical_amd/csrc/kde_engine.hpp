@@ -47,7 +47,14 @@ struct DJob {
   uint64_t pbase;    // first slot in the concatenated point arrays
   uint64_t out_off;  // first density slot in the output array
   double norm;       // prod_e(0.75/h_e) / n_rows
+  uint64_t cells;    // prod_e nb[e]
+  uint64_t cbase;    // first entry of this job's cell-start table, or kNoTable
 };
+
+// Jobs whose cell count exceeds this multiple of their points (sparse grids,
+// heavy tails) locate candidate ranges by binary search instead of a table.
+constexpr uint64_t kNoTable = ~0ull;
+constexpr uint64_t kTableCellsPerPoint = 8;
 
 // Tile: queries [p0, p1) of job `job` (sorted positions, job-local).
 struct DTile {

@@ -8,6 +8,16 @@
 #include "kde_engine.hpp"
 #include "kde_kernels.hpp"
 
+// Register budget of the density kernel: at least this many waves per SIMD
+// (HIP's second launch-bounds argument), i.e. at most 512/N VGPRs per lane.
+#ifndef ENT_DENSITY_MIN_WAVES
+#define ENT_DENSITY_MIN_WAVES 1
+#endif
+// LDS candidate reads issued ahead of their first use in the inner loop.
+#ifndef ENT_LDS_AHEAD
+#define ENT_LDS_AHEAD 1
+#endif
+
 namespace ent {
 namespace {
 
@@ -124,10 +134,12 @@ __global__ __launch_bounds__(kBlock) void k_minmax(const DJob* __restrict__ jobs
 }
 
 // ------------------------------------------------------------- cell keys
+// KeyT is uint32_t when every key of the batch fits 32 bits, else uint64_t.
+template <typename KeyT>
 __global__ __launch_bounds__(kBlock) void k_cell_keys(const DJob* __restrict__ jobs,
                                                       const uint2* __restrict__ chunks,
                                                       uint32_t n_chunks, const float* __restrict__ X,
-                                                      uint64_t* __restrict__ keys,
+                                                      KeyT* __restrict__ keys,
                                                       uint32_t* __restrict__ idx) {
   const uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (c >= n_chunks) return;
@@ -139,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(const DJob* __restrict__ j
     if (k < J.n) {
       float v[3] = {0.f, 0.f, 0.f};
       for (uint32_t e = 0; e < J.d; ++e) v[e] = X[J.off[e] + k];
-      keys[J.pbase + k] = cell_key(J, v);
+      keys[J.pbase + k] = (KeyT)cell_key(J, v);
       idx[J.pbase + k] = k;
     }
   }
@@ -159,11 +171,12 @@ __global__ void k_bin_ids(const DJob* __restrict__ jobs, const float* __restrict
 // one global exclusive scan over the whole array yields, for every
 // (job, digit, chunk), the global destination of that chunk's first element
 // with that digit (jobs are laid out back to back at pbase).
+template <typename KeyT>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const DJob* __restrict__ jobs,
                                                        const uint2* __restrict__ chunks,
                                                        const uint32_t* __restrict__ chunk_first,
                                                        uint32_t n_chunks,
-                                                       const uint64_t* __restrict__ keys, int shift,
+                                                       const KeyT* __restrict__ keys, int shift,
                                                        uint32_t* __restrict__ hist) {
   __shared__ uint32_t bins[kBlock / kWave][kRadixBins];
   const int w = threadIdx.x / kWave;
@@ -189,13 +202,16 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const DJob* __restrict__ 
 }
 
 // Stable scatter: a wave walks its chunk in order, 64 elements at a time;
-// equal digits inside a step are ranked by lane via 8 ballots.
+// equal digits inside a step are ranked by lane via 8 ballots.  With GATHER
+// (last pass) the point's coordinates are written to the sorted SoA array at
+// the same destination, so no separate gather pass re-reads the permutation.
+template <typename KeyT, bool GATHER>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const DJob* __restrict__ jobs, const uint2* __restrict__ chunks,
     const uint32_t* __restrict__ chunk_first, uint32_t n_chunks,
-    const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ idx_in,
-    uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out, int shift,
-    const uint32_t* __restrict__ offs) {
+    const KeyT* __restrict__ keys_in, const uint32_t* __restrict__ idx_in,
+    KeyT* __restrict__ keys_out, uint32_t* __restrict__ idx_out, int shift,
+    const uint32_t* __restrict__ offs, const float* __restrict__ X, float* __restrict__ scoord) {
   __shared__ uint32_t run[kBlock / kWave][kRadixBins];
   const int w = threadIdx.x / kWave;
   const int l = lane_id();
@@ -212,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
   for (int i = 0; i < kChunkItems; ++i) {
     uint32_t k = ch.y + i * kWave + l;
     bool valid = k < J.n;
-    uint64_t key = 0;
+    KeyT key = 0;
     uint32_t id = 0, dg = 0;
     if (valid) {
       key = keys_in[J.pbase + k];
@@ -235,6 +251,10 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
       uint32_t dst = base + rank;
       keys_out[dst] = key;
       idx_out[dst] = id;
+      if constexpr (GATHER) {
+        const uint64_t p = dst - J.pbase;
+        for (uint32_t e = 0; e < J.d; ++e) scoord[3 * J.pbase + (uint64_t)e * J.n + p] = X[J.off[e] + id];
+      }
     }
   }
 }
@@ -310,13 +330,40 @@ __global__ __launch_bounds__(kBlock) void k_gather_sorted(const DJob* __restrict
   }
 }
 
+// ------------------------------------------------------------ cell starts
+// start[cbase + c] = first sorted position whose key >= c, for c in
+// [0, cells]; each position fills the gap of empty cells before its own key.
+template <typename KeyT>
+__global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__ jobs,
+                                                        const uint2* __restrict__ chunks,
+                                                        uint32_t n_chunks, const KeyT* __restrict__ keys,
+                                                        uint32_t* __restrict__ start) {
+  const uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (c >= n_chunks) return;
+  const uint2 ch = chunks[c];
+  const DJob& J = jobs[ch.x];
+  if (J.cbase == kNoTable) return;
+  const int l = lane_id();
+  uint32_t* st = start + J.cbase;
+  const KeyT* kk = keys + J.pbase;
+  for (int i = 0; i < kChunkItems; ++i) {
+    uint32_t p = ch.y + i * kWave + l;
+    if (p >= J.n) break;
+    const uint64_t cur = kk[p];
+    const uint64_t first = p == 0 ? 0 : (uint64_t)kk[p - 1] + 1;
+    for (uint64_t cell = first; cell <= cur; ++cell) st[cell] = p;
+    if (p + 1 == J.n)
+      for (uint64_t cell = cur + 1; cell <= J.cells; ++cell) st[cell] = J.n;
+  }
+}
+
 // ------------------------------------------------------------------ tiles
 // A tile starts at sorted position p when p == 0, p % tq == 0, or the
 // (row, dim-0 group) of p differs from that of p-1.  Tiles never cross rows,
 // have at most tq queries and span at most `group` dim-0 boxes.
 namespace {
-__device__ __forceinline__ bool tile_start(const DJob& J, const uint64_t* keys, uint32_t p,
-                                           uint32_t tq) {
+template <typename KeyT>
+__device__ __forceinline__ bool tile_start(const DJob& J, const KeyT* keys, uint32_t p, uint32_t tq) {
   if (p == 0 || (p % tq) == 0) return true;
   if (J.flags & kJobBrute) return false;
   uint64_t a = keys[J.pbase + p - 1], b = keys[J.pbase + p];
@@ -326,10 +373,11 @@ __device__ __forceinline__ bool tile_start(const DJob& J, const uint64_t* keys, 
 }
 }  // namespace
 
+template <typename KeyT>
 __global__ __launch_bounds__(kBlock) void k_tile_count(const DJob* __restrict__ jobs,
                                                        const uint2* __restrict__ chunks,
                                                        uint32_t n_chunks,
-                                                       const uint64_t* __restrict__ keys,
+                                                       const KeyT* __restrict__ keys,
                                                        uint32_t tq, uint32_t* __restrict__ cnt) {
   const uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (c >= n_chunks) return;
@@ -345,10 +393,11 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(const DJob* __restrict__ 
   if (l == 0) cnt[c] = total;
 }
 
+template <typename KeyT>
 __global__ __launch_bounds__(kBlock) void k_tile_write(const DJob* __restrict__ jobs,
                                                        const uint2* __restrict__ chunks,
                                                        uint32_t n_chunks,
-                                                       const uint64_t* __restrict__ keys,
+                                                       const KeyT* __restrict__ keys,
                                                        uint32_t tq,
                                                        const uint32_t* __restrict__ offs,
                                                        DTile* __restrict__ tiles) {
@@ -397,11 +446,13 @@ constexpr float kSentinel = 1.0e18f;
 
 __device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
 
-__device__ __forceinline__ uint32_t lower_bound_u64(const uint64_t* a, uint32_t n, uint64_t v) {
+// First position in a[0..n) whose key is >= v (keys compared as uint64).
+template <typename KeyT>
+__device__ __forceinline__ uint32_t lower_bound_key(const KeyT* a, uint32_t n, uint64_t v) {
   uint32_t lo = 0, hi = n;
   while (lo < hi) {
     uint32_t mid = (lo + hi) >> 1;
-    if (a[mid] < v) lo = mid + 1;
+    if ((uint64_t)a[mid] < v) lo = mid + 1;
     else hi = mid;
   }
   return lo;
@@ -454,50 +505,60 @@ __device__ __forceinline__ void accumulate_chunk(const WaveLds& L, const float (
 #pragma unroll
   for (int q = 0; q < Q; ++q) part[q] = 0.0f;
   constexpr int kPer4 = (D == 1) ? 4 : (D == 2 ? 2 : 1);
+  constexpr int kReads = kWave / kPer4;  // float4 LDS reads per chunk
+  constexpr int kAhead = D == 3 ? 1 : ENT_LDS_AHEAD;  // reads issued before the first use
 #pragma unroll
-  for (int j4 = 0; j4 < kWave / kPer4; ++j4) {
-    float4 v = c4[j4];
-    float cs[4][3];
-    if constexpr (D == 1) {
-      cs[0][0] = v.x; cs[1][0] = v.y; cs[2][0] = v.z; cs[3][0] = v.w;
-    } else if constexpr (D == 2) {
-      cs[0][0] = v.x; cs[0][1] = v.y; cs[1][0] = v.z; cs[1][1] = v.w;
-    } else {
-      cs[0][0] = v.x; cs[0][1] = v.y; cs[0][2] = v.z;
-    }
+  for (int g = 0; g < kReads; g += kAhead) {
+    float4 vv[kAhead];
 #pragma unroll
-    for (int c = 0; c < kPer4; ++c) {
+    for (int u = 0; u < kAhead; ++u) vv[u] = c4[g + u];
 #pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        float w[3];
-#pragma unroll
-        for (int e = 0; e < D; ++e) {
-          float u = sq[e][q] - cs[c][e];
-          if constexpr (RAW) u = u * hinv[e];
-          if constexpr (TRI) w[e] = clamp01(1.0f - fabsf(u));
-          else w[e] = clamp01(__builtin_fmaf(-u, u, 1.0f));
-        }
-        if constexpr (D == 1) part[q] += w[0];
-        else if constexpr (D == 2) part[q] = __builtin_fmaf(w[0], w[1], part[q]);
-        else part[q] = __builtin_fmaf(w[0] * w[1], w[2], part[q]);
+    for (int u = 0; u < kAhead; ++u) {
+      const float4 v = vv[u];
+      float cs[4][3];
+      if constexpr (D == 1) {
+        cs[0][0] = v.x; cs[1][0] = v.y; cs[2][0] = v.z; cs[3][0] = v.w;
+      } else if constexpr (D == 2) {
+        cs[0][0] = v.x; cs[0][1] = v.y; cs[1][0] = v.z; cs[1][1] = v.w;
+      } else {
+        cs[0][0] = v.x; cs[0][1] = v.y; cs[0][2] = v.z;
       }
-    }
-    // compensated merge every 32 candidates
-    if (((j4 + 1) * kPer4) % 32 == 0) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        kahan_add(S[q], C[q], part[q]);
-        part[q] = 0.0f;
+      for (int c = 0; c < kPer4; ++c) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          float w[3];
+#pragma unroll
+          for (int e = 0; e < D; ++e) {
+            float uu = sq[e][q] - cs[c][e];
+            if constexpr (RAW) uu = uu * hinv[e];
+            if constexpr (TRI) w[e] = clamp01(1.0f - fabsf(uu));
+            else w[e] = clamp01(__builtin_fmaf(-uu, uu, 1.0f));
+          }
+          if constexpr (D == 1) part[q] += w[0];
+          else if constexpr (D == 2) part[q] = __builtin_fmaf(w[0], w[1], part[q]);
+          else part[q] = __builtin_fmaf(w[0] * w[1], w[2], part[q]);
+        }
+      }
+      // compensated merge every 32 candidates
+      if (((g + u + 1) * kPer4) % 32 == 0) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          kahan_add(S[q], C[q], part[q]);
+          part[q] = 0.0f;
+        }
       }
     }
   }
 }
 
-template <int D, int Q, bool RAW>
-__device__ void process_tile(const DJob& J, const DTile& T, const float* __restrict__ X,
-                             const float* __restrict__ scoord, const uint64_t* __restrict__ keys,
-                             const uint32_t* __restrict__ idx, double* __restrict__ dens,
-                             unsigned long long* __restrict__ pairs, WaveLds& L) {
+// Densities of one tile's queries; returns the tile's candidate count
+// (wave-uniform).
+template <int D, int Q, bool RAW, typename KeyT>
+__device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __restrict__ X,
+                                 const float* __restrict__ scoord, const KeyT* __restrict__ keys,
+                                 const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ idx,
+                                 double* __restrict__ dens, WaveLds& L) {
   const int l = lane_id();
   const uint32_t n = J.n;
   const float* src[3];
@@ -552,9 +613,16 @@ __device__ void process_tile(const DJob& J, const DTile& T, const float* __restr
       uint64_t row = (uint64_t)r2 * J.nb[1] + r1;
       uint64_t klo = row * J.nb[0] + lo[0];
       uint64_t khi = row * J.nb[0] + hi[0];
-      const uint64_t* kk = keys + J.pbase;
-      uint32_t a = lower_bound_u64(kk, n, klo);
-      uint32_t b = lower_bound_u64(kk, n, khi + 1);
+      uint32_t a, b;
+      if (J.cbase != kNoTable) {
+        const uint32_t* st = cstart + J.cbase;
+        a = st[klo];
+        b = st[khi + 1];
+      } else {
+        const KeyT* kk = keys + J.pbase;
+        a = lower_bound_key(kk, n, klo);
+        b = a + lower_bound_key(kk + a, n - a, khi + 1);
+      }
       my_lo = a;
       my_len = b - a;
     }
@@ -563,10 +631,11 @@ __device__ void process_tile(const DJob& J, const DTile& T, const float* __restr
       // row between the first and last candidate row (a superset).
       uint64_t row_a = (uint64_t)lo[2] * J.nb[1] + lo[1];
       uint64_t row_b = (uint64_t)hi[2] * J.nb[1] + hi[1];
-      const uint64_t* kk = keys + J.pbase;
+      const KeyT* kk = keys + J.pbase;
       if (l == 0) {
-        uint32_t a = lower_bound_u64(kk, n, row_a * J.nb[0]);
-        uint32_t b = lower_bound_u64(kk, n, (row_b + 1) * J.nb[0]);
+        const bool tab = J.cbase != kNoTable;
+        uint32_t a = tab ? cstart[J.cbase + row_a * J.nb[0]] : lower_bound_key(kk, n, row_a * J.nb[0]);
+        uint32_t b = tab ? cstart[J.cbase + (row_b + 1) * J.nb[0]] : lower_bound_key(kk, n, (row_b + 1) * J.nb[0]);
         my_lo = a;
         my_len = b - a;
       }
@@ -576,8 +645,7 @@ __device__ void process_tile(const DJob& J, const DTile& T, const float* __restr
 
   // flattened candidate index space over the ranges
   uint32_t incl = wave_incl_scan(my_len);
-  uint32_t total = __shfl(incl, kWave - 1, kWave);
-  if (l == 0) atomicAdd(pairs + (D - 1), (unsigned long long)(T.p1 - T.p0) * total);
+  const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, kWave - 1, kWave));
   if ((uint32_t)l < n_ranges) {
     L.r_lo[l] = my_lo;
     L.r_ps[l] = incl - my_len;
@@ -645,20 +713,44 @@ __device__ void process_tile(const DJob& J, const DTile& T, const float* __restr
       dens[J.out_off + k] = ((double)S[q] - (double)C[q]) * J.norm;
     }
   }
+  return total;
+}
+
+// Queries per lane chosen per tile: the smallest Q <= QMAX whose 64*Q lane
+// slots hold the tile, so partly filled tiles do not pay for idle slots.
+template <int D, int QMAX, bool RAW, typename KeyT>
+__device__ __forceinline__ void run_tile(const DJob& J, const DTile& T, const float* __restrict__ X,
+                                         const float* __restrict__ scoord, const KeyT* __restrict__ keys,
+                                         const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ idx,
+                                         double* __restrict__ dens, WaveLds& L, uint64_t& pairs,
+                                         uint64_t& lane_slots) {
+  const uint32_t nq = T.p1 - T.p0;
+#ifndef ENT_STATIC_Q
+  if constexpr (QMAX >= 2 && !RAW) {  // plain-path tiles are full except the last
+    if (nq <= (uint32_t)(kWave * (QMAX / 2))) {
+      run_tile<D, QMAX / 2, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, pairs, lane_slots);
+      return;
+    }
+  }
+#endif
+  const uint32_t total = process_tile<D, QMAX, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L);
+  pairs += (uint64_t)nq * total;
+  lane_slots += (uint64_t)(kWave * QMAX) * ((total + kWave - 1) / kWave * kWave);
 }
 
 }  // namespace
 
 // Persistent density kernel: every wave pulls tiles from a global counter
 // until the tile list is exhausted.
-template <int Q, bool RAW>
-__global__ __launch_bounds__(kBlock) void k_density(const DJob* __restrict__ jobs,
+template <int Q, bool RAW, typename KeyT>
+__global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const DJob* __restrict__ jobs,
                                                     const DTile* __restrict__ tiles,
                                                     const uint32_t* __restrict__ n_tiles_p,
                                                     uint32_t* __restrict__ counter,
                                                     const float* __restrict__ X,
                                                     const float* __restrict__ scoord,
-                                                    const uint64_t* __restrict__ keys,
+                                                    const KeyT* __restrict__ keys,
+                                                    const uint32_t* __restrict__ cstart,
                                                     const uint32_t* __restrict__ idx,
                                                     double* __restrict__ dens,
                                                     unsigned long long* __restrict__ pairs) {
@@ -666,6 +758,8 @@ __global__ __launch_bounds__(kBlock) void k_density(const DJob* __restrict__ job
   WaveLds& L = lds[threadIdx.x / kWave];
   const uint32_t n_tiles = *n_tiles_p;
   const int l = lane_id();
+  // executed pairs per dimension and lane slots, flushed once per wave
+  uint64_t acc[3] = {0, 0, 0}, slots = 0;
   for (;;) {
     uint32_t t = 0;
     if (l == 0) t = atomicAdd(counter, 1u);
@@ -674,9 +768,14 @@ __global__ __launch_bounds__(kBlock) void k_density(const DJob* __restrict__ job
     const DTile T = tiles[t];
     const DJob J = jobs[T.job];
     const uint32_t d = __builtin_amdgcn_readfirstlane(J.d);
-    if (d == 1) process_tile<1, Q, RAW>(J, T, X, scoord, keys, idx, dens, pairs, L);
-    else if (d == 2) process_tile<2, Q, RAW>(J, T, X, scoord, keys, idx, dens, pairs, L);
-    else process_tile<3, Q, RAW>(J, T, X, scoord, keys, idx, dens, pairs, L);
+    if (d == 1) run_tile<1, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[0], slots);
+    else if (d == 2) run_tile<2, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[1], slots);
+    else run_tile<3, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[2], slots);
+  }
+  if (l == 0) {
+    for (int e = 0; e < 3; ++e)
+      if (acc[e]) atomicAdd(pairs + e, (unsigned long long)acc[e]);
+    if (slots) atomicAdd(pairs + 3, (unsigned long long)slots);
   }
 }
 
@@ -702,9 +801,14 @@ void launch_minmax(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_
   hipLaunchKernelGGL(k_minmax, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks, n_chunks, X, mm, bad);
 }
 
-void launch_cell_keys(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                      const float* X, uint64_t* keys, uint32_t* idx) {
-  hipLaunchKernelGGL(k_cell_keys, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks, n_chunks, X, keys, idx);
+void launch_cell_keys(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
+                      const float* X, void* keys, uint32_t* idx) {
+  if (wide)
+    hipLaunchKernelGGL(k_cell_keys<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks, n_chunks,
+                       X, (uint64_t*)keys, idx);
+  else
+    hipLaunchKernelGGL(k_cell_keys<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks, n_chunks,
+                       X, (uint32_t*)keys, idx);
 }
 
 void launch_bin_ids(hipStream_t s, const DJob* jobs, const float* X, int32_t* box, uint32_t n) {
@@ -714,18 +818,33 @@ void launch_bin_ids(hipStream_t s, const DJob* jobs, const float* X, int32_t* bo
   hipLaunchKernelGGL(k_bin_ids, dim3(blocks), dim3(kBlock), 0, s, jobs, X, box);
 }
 
-void launch_radix_hist(hipStream_t s, const DJob* jobs, const uint2* chunks, const uint32_t* chunk_first,
-                       uint32_t n_chunks, const uint64_t* keys, int shift, uint32_t* hist) {
-  hipLaunchKernelGGL(k_radix_hist, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks, chunk_first,
-                     n_chunks, keys, shift, hist);
+void launch_radix_hist(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks,
+                       const uint32_t* chunk_first, uint32_t n_chunks, const void* keys, int shift,
+                       uint32_t* hist) {
+  if (wide)
+    hipLaunchKernelGGL(k_radix_hist<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
+                       chunk_first, n_chunks, (const uint64_t*)keys, shift, hist);
+  else
+    hipLaunchKernelGGL(k_radix_hist<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
+                       chunk_first, n_chunks, (const uint32_t*)keys, shift, hist);
 }
 
-void launch_radix_scatter(hipStream_t s, const DJob* jobs, const uint2* chunks,
-                          const uint32_t* chunk_first, uint32_t n_chunks, const uint64_t* kin,
-                          const uint32_t* iin, uint64_t* kout, uint32_t* iout, int shift,
-                          const uint32_t* offs) {
-  hipLaunchKernelGGL(k_radix_scatter, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks, chunk_first,
-                     n_chunks, kin, iin, kout, iout, shift, offs);
+void launch_radix_scatter(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks,
+                          const uint32_t* chunk_first, uint32_t n_chunks, const void* kin,
+                          const uint32_t* iin, void* kout, uint32_t* iout, int shift, const uint32_t* offs,
+                          const float* X, float* scoord) {
+  const bool gather = scoord != nullptr;
+#define ENT_SCATTER(KT, G)                                                                                   \
+  hipLaunchKernelGGL((k_radix_scatter<KT, G>), chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,       \
+                     chunk_first, n_chunks, (const KT*)kin, iin, (KT*)kout, iout, shift, offs, X, scoord)
+  if (wide) {
+    if (gather) ENT_SCATTER(uint64_t, true);
+    else ENT_SCATTER(uint64_t, false);
+  } else {
+    if (gather) ENT_SCATTER(uint32_t, true);
+    else ENT_SCATTER(uint32_t, false);
+  }
+#undef ENT_SCATTER
 }
 
 uint64_t scan_scratch_elems(uint64_t m) {
@@ -762,16 +881,24 @@ void launch_gather_sorted(hipStream_t s, const DJob* jobs, const uint2* chunks, 
                      idx, scoord);
 }
 
-void launch_tile_count(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const uint64_t* keys, uint32_t tq, uint32_t* cnt) {
-  hipLaunchKernelGGL(k_tile_count, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks, n_chunks, keys,
-                     tq, cnt);
+void launch_tile_count(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
+                       const void* keys, uint32_t tq, uint32_t* cnt) {
+  if (wide)
+    hipLaunchKernelGGL(k_tile_count<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
+                       n_chunks, (const uint64_t*)keys, tq, cnt);
+  else
+    hipLaunchKernelGGL(k_tile_count<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
+                       n_chunks, (const uint32_t*)keys, tq, cnt);
 }
 
-void launch_tile_write(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const uint64_t* keys, uint32_t tq, const uint32_t* offs, DTile* tiles) {
-  hipLaunchKernelGGL(k_tile_write, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks, n_chunks, keys,
-                     tq, offs, tiles);
+void launch_tile_write(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
+                       const void* keys, uint32_t tq, const uint32_t* offs, DTile* tiles) {
+  if (wide)
+    hipLaunchKernelGGL(k_tile_write<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
+                       n_chunks, (const uint64_t*)keys, tq, offs, tiles);
+  else
+    hipLaunchKernelGGL(k_tile_write<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
+                       n_chunks, (const uint32_t*)keys, tq, offs, tiles);
 }
 
 void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
@@ -782,40 +909,67 @@ void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint3
   hipLaunchKernelGGL(k_tile_ends, dim3((unsigned)blocks), dim3(kBlock), 0, s, jobs, tiles, total);
 }
 
-template <int Q, bool RAW>
-static const void* density_fn() {
-  return reinterpret_cast<const void*>(&k_density<Q, RAW>);
+// Kernel variants: RAW (plain path, keys unused) or cell lists with 32- or
+// 64-bit keys, for q = 1, 2, 4, 8 queries per lane.
+template <int Q>
+static const void* density_fn(bool raw, bool wide) {
+  if (raw) return reinterpret_cast<const void*>(&k_density<Q, true, uint32_t>);
+  if (wide) return reinterpret_cast<const void*>(&k_density<Q, false, uint64_t>);
+  return reinterpret_cast<const void*>(&k_density<Q, false, uint32_t>);
 }
 
-int density_occupancy(int q, bool raw, int* blocks_per_cu) {
+int density_occupancy(int q, bool raw, bool wide, int* blocks_per_cu) {
   const void* fn = nullptr;
   switch (q) {
-    case 1: fn = raw ? density_fn<1, true>() : density_fn<1, false>(); break;
-    case 2: fn = raw ? density_fn<2, true>() : density_fn<2, false>(); break;
-    case 4: fn = raw ? density_fn<4, true>() : density_fn<4, false>(); break;
-    case 8: fn = raw ? density_fn<8, true>() : density_fn<8, false>(); break;
+    case 1: fn = density_fn<1>(raw, wide); break;
+    case 2: fn = density_fn<2>(raw, wide); break;
+    case 4: fn = density_fn<4>(raw, wide); break;
     default: return -1;
   }
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, kBlock, 0);
   return e == hipSuccess ? 0 : -1;
 }
 
-int launch_density(hipStream_t s, int q, bool raw, unsigned grid, const DJob* jobs, const DTile* tiles,
-                   const uint32_t* n_tiles, uint32_t* counter, const float* X, const float* scoord,
-                   const uint64_t* keys, const uint32_t* idx, double* dens,
-                   unsigned long long* pairs) {
-#define ENT_LAUNCH_DENSITY(QQ, RR)                                                                    \
-  hipLaunchKernelGGL((k_density<QQ, RR>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles, counter, \
-                     X, scoord, keys, idx, dens, pairs)
+template <int Q>
+static void launch_density_q(hipStream_t s, bool raw, bool wide, unsigned grid, const DJob* jobs,
+                             const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
+                             const float* scoord, const void* keys, const uint32_t* cstart,
+                             const uint32_t* idx, double* dens, unsigned long long* pairs) {
+  if (raw)
+    hipLaunchKernelGGL((k_density<Q, true, uint32_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
+                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, dens, pairs);
+  else if (wide)
+    hipLaunchKernelGGL((k_density<Q, false, uint64_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
+                       counter, X, scoord, (const uint64_t*)keys, cstart, idx, dens, pairs);
+  else
+    hipLaunchKernelGGL((k_density<Q, false, uint32_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
+                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, dens, pairs);
+}
+
+int launch_density(hipStream_t s, int q, bool raw, bool wide, unsigned grid, const DJob* jobs,
+                   const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
+                   const float* scoord, const void* keys, const uint32_t* cstart, const uint32_t* idx,
+                   double* dens, unsigned long long* pairs) {
+#define ENT_LD(QQ) \
+  launch_density_q<QQ>(s, raw, wide, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart, idx, dens, pairs)
   switch (q) {
-    case 1: if (raw) ENT_LAUNCH_DENSITY(1, true); else ENT_LAUNCH_DENSITY(1, false); break;
-    case 2: if (raw) ENT_LAUNCH_DENSITY(2, true); else ENT_LAUNCH_DENSITY(2, false); break;
-    case 4: if (raw) ENT_LAUNCH_DENSITY(4, true); else ENT_LAUNCH_DENSITY(4, false); break;
-    case 8: if (raw) ENT_LAUNCH_DENSITY(8, true); else ENT_LAUNCH_DENSITY(8, false); break;
+    case 1: ENT_LD(1); break;
+    case 2: ENT_LD(2); break;
+    case 4: ENT_LD(4); break;
     default: return -1;
   }
-#undef ENT_LAUNCH_DENSITY
+#undef ENT_LD
   return 0;
+}
+
+void launch_cell_starts(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
+                        const void* keys, uint32_t* start) {
+  if (wide)
+    hipLaunchKernelGGL(k_cell_starts<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
+                       n_chunks, (const uint64_t*)keys, start);
+  else
+    hipLaunchKernelGGL(k_cell_starts<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
+                       n_chunks, (const uint32_t*)keys, start);
 }
 
 }  // namespace ent

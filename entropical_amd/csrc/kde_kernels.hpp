@@ -1,4 +1,5 @@
 // kde_kernels.hpp -- host-side launchers of the kernels in kde_kernels.hip.
+// `wide` selects 64-bit cell keys (else 32-bit); key buffers are void*.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -10,30 +11,34 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint64_t n);
 void launch_init_minmax(hipStream_t s, uint32_t* mm, uint32_t n_jobs);
 void launch_minmax(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
                    const float* X, uint32_t* mm, uint32_t* bad);
-void launch_cell_keys(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                      const float* X, uint64_t* keys, uint32_t* idx);
+void launch_cell_keys(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
+                      const float* X, void* keys, uint32_t* idx);
 void launch_bin_ids(hipStream_t s, const DJob* jobs, const float* X, int32_t* box, uint32_t n);
-void launch_radix_hist(hipStream_t s, const DJob* jobs, const uint2* chunks, const uint32_t* chunk_first,
-                       uint32_t n_chunks, const uint64_t* keys, int shift, uint32_t* hist);
-void launch_radix_scatter(hipStream_t s, const DJob* jobs, const uint2* chunks,
-                          const uint32_t* chunk_first, uint32_t n_chunks, const uint64_t* kin,
-                          const uint32_t* iin, uint64_t* kout, uint32_t* iout, int shift,
-                          const uint32_t* offs);
+void launch_radix_hist(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks,
+                       const uint32_t* chunk_first, uint32_t n_chunks, const void* keys, int shift,
+                       uint32_t* hist);
+// scoord != nullptr: also write sorted coordinates (last pass)
+void launch_radix_scatter(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks,
+                          const uint32_t* chunk_first, uint32_t n_chunks, const void* kin,
+                          const uint32_t* iin, void* kout, uint32_t* iout, int shift, const uint32_t* offs,
+                          const float* X, float* scoord);
 uint64_t scan_scratch_elems(uint64_t m);
 void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t m,
                            uint32_t* scratch);
 void launch_gather_sorted(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
                           const float* X, const uint32_t* idx, float* scoord);
-void launch_tile_count(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const uint64_t* keys, uint32_t tq, uint32_t* cnt);
-void launch_tile_write(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const uint64_t* keys, uint32_t tq, const uint32_t* offs, DTile* tiles);
+void launch_tile_count(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
+                       const void* keys, uint32_t tq, uint32_t* cnt);
+void launch_tile_write(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
+                       const void* keys, uint32_t tq, const uint32_t* offs, DTile* tiles);
 void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
                       uint64_t max_tiles);
-int density_occupancy(int q, bool raw, int* blocks_per_cu);
-int launch_density(hipStream_t s, int q, bool raw, unsigned grid, const DJob* jobs, const DTile* tiles,
-                   const uint32_t* n_tiles, uint32_t* counter, const float* X, const float* scoord,
-                   const uint64_t* keys, const uint32_t* idx, double* dens,
-                   unsigned long long* pairs);
+int density_occupancy(int q, bool raw, bool wide, int* blocks_per_cu);
+int launch_density(hipStream_t s, int q, bool raw, bool wide, unsigned grid, const DJob* jobs,
+                   const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
+                   const float* scoord, const void* keys, const uint32_t* cstart, const uint32_t* idx,
+                   double* dens, unsigned long long* pairs);
+void launch_cell_starts(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
+                        const void* keys, uint32_t* start);
 
 }  // namespace ent

@@ -75,10 +75,10 @@ class Context:
         dm, tm = C.c_double(), C.c_double()
         pairs, tiles = C.c_uint64(), C.c_uint64()
         check(load().ent_ctx_last_stats(self.handle, C.byref(dm), C.byref(tm), C.byref(pairs), C.byref(tiles)))
-        by_d = (C.c_uint64 * 3)()
+        by_d = (C.c_uint64 * 4)()
         check(load().ent_ctx_last_pairs(self.handle, by_d))
         return {"density_ms": dm.value, "total_ms": tm.value, "pairs": pairs.value, "tiles": tiles.value,
-                "pairs_by_d": [int(v) for v in by_d]}
+                "pairs_by_d": [int(v) for v in by_d[:3]], "lane_pairs": int(by_d[3])}
 
     # device memory -------------------------------------------------------
     def alloc(self, nbytes: int) -> int:

@@ -66,7 +66,7 @@ int ent_device_count(int* n);
 int ent_ctx_create(int device, ent_ctx** out);
 int ent_ctx_destroy(ent_ctx* ctx);
 
-/* Engine tuning: queries per lane (1,2,4,8), dim-0 boxes per bandwidth,
+/* Engine tuning: maximum queries per lane (1,2,4; smaller tiles use fewer), dim-0 boxes per bandwidth,
  * row-dim boxes per bandwidth, tile extent bound in bandwidths.  Replaces
  * the reference's compile-time BSIZE (densities_cuda.cu:14). */
 int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h);
@@ -83,8 +83,10 @@ int ent_ctx_last_stats(ent_ctx* ctx, double* density_ms, double* total_ms, uint6
 int ent_ctx_set_graph(ent_ctx* ctx, int enable);
 
 /* Executed pair evaluations of the last density run, split by job dimension
- * (out[d-1] for d = 1, 2, 3). */
-int ent_ctx_last_pairs(ent_ctx* ctx, uint64_t* out3);
+ * (out[d-1] for d = 1, 2, 3), and out[3] = lane-slot evaluations issued
+ * (64*Q lanes per tile times candidates rounded up to chunks; the ratio
+ * sum(out[0..2]) / out[3] is the lane utilisation). */
+int ent_ctx_last_pairs(ent_ctx* ctx, uint64_t* out4);
 
 /* Device memory helpers on the context's device and stream. */
 int ent_dev_alloc(ent_ctx* ctx, size_t bytes, void** ptr);
@@ -167,6 +169,28 @@ int ent_batch_entropy_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs,
 
 /* Wall time (ms) of the last driver call on this context. */
 int ent_ctx_last_driver_ms(ent_ctx* ctx, double* ms);
+
+/* ---- learned bandwidth model (no reference counterpart; the reference
+ * chooses h with Hestimate::Thumb / AmiseMin, hestimate.cpp:19-104, which
+ * stay available as the fallback and the judge).  A multilayer perceptron
+ * n_in -> hidden^n_hidden_layers -> 1 with tanh units, BF16 weights and FP32
+ * accumulation, trained with minibatch Adam by this library's kernels. */
+typedef struct ent_mlp ent_mlp;
+int ent_mlp_create(ent_ctx* ctx, int n_in, int hidden, int n_hidden_layers, uint64_t seed, ent_mlp** out);
+int ent_mlp_destroy(ent_mlp* mlp);
+int ent_mlp_num_params(ent_mlp* mlp, size_t* n);
+/* FP32 master parameters: per layer W[n_in][n_out] then b[n_out]. */
+int ent_mlp_get_params(ent_ctx* ctx, ent_mlp* mlp, float* out);
+int ent_mlp_set_params(ent_ctx* ctx, ent_mlp* mlp, const float* in);
+/* y[i] = mlp(X[i, :]) for n rows of n_in floats (host arrays). */
+int ent_mlp_predict(ent_ctx* ctx, ent_mlp* mlp, const float* X, size_t n, float* y);
+/* Gradient of the batch mean squared error at the current parameters. */
+int ent_mlp_gradient(ent_ctx* ctx, ent_mlp* mlp, const float* X, const float* t, size_t n, float* grads,
+                     float* loss);
+/* Minibatch Adam: order holds epochs*n row indices (one permutation per
+ * epoch); losses receives the mean training loss of each epoch. */
+int ent_mlp_train(ent_ctx* ctx, ent_mlp* mlp, const float* X, const float* t, size_t n,
+                  const uint32_t* order, int epochs, int batch, float lr, float* losses);
 
 #ifdef __cplusplus
 }
