@@ -7,7 +7,10 @@ from ._lib import EntropicalError  # noqa: F401
 from .kde import (  # noqa: F401
     Context,
     DeviceArray,
+    batch_autotune,
     batch_entropy,
+    mi_autotune,
+    te_autotune,
     last_driver_ms,
     mi_matrix,
     te_matrix,
@@ -37,4 +40,7 @@ __all__ = [
     "te_matrix",
     "batch_entropy",
     "last_driver_ms",
+    "mi_autotune",
+    "te_autotune",
+    "batch_autotune",
 ]

@@ -85,6 +85,11 @@ SIGNATURES = {
     "ent_te_matrix_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P, C.c_size_t, C.c_int, C.c_int, P]),
     "ent_batch_entropy_dev": (C.c_int, [P, P, C.POINTER(ent_job), C.c_size_t, C.c_int, P]),
     "ent_ctx_last_driver_ms": (C.c_int, [P, F64P]),
+    "ent_batch_loo_dev": (C.c_int, [P, P, C.POINTER(ent_job), C.c_size_t, C.c_double, C.c_int, P]),
+    "ent_mi_autotune_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P]),
+    "ent_te_autotune_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P, C.c_size_t, P]),
+    "ent_autotune_dev": (C.c_int, [P, P, C.POINTER(ent_job), C.c_size_t, P, F64P]),
+    "ent_ctx_get_tuning": (C.c_int, [P, P]),
     "ent_mlp_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]),
     "ent_mlp_destroy": (C.c_int, [P]),
     "ent_mlp_num_params": (C.c_int, [P, C.POINTER(C.c_size_t)]),

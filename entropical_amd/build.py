@@ -22,7 +22,8 @@ BUILD = ROOT / "_build"
 LIB = PKG / "libentropical_kde.so"
 ARCH = os.environ.get("ENT_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kde_kernels.hip", "engine.hip", "capi.hip", "reductions.hip", "drivers.hip", "mlp.hip"]
+HIP_SOURCES = ["kde_kernels.hip", "engine.hip", "autotune.hip", "capi.hip", "reductions.hip", "drivers.hip",
+               "mlp.hip"]
 HOST_SOURCES = ["densities_shim.cpp"]
 
 HIPCC_FLAGS = [

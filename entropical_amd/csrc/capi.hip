@@ -154,6 +154,20 @@ int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h) {
   return ENT_OK;
 }
 
+int ent_ctx_get_tuning(ent_ctx* ctx, int* tuning4) {
+  CAPI_TRY(check_ctx(ctx));
+  if (!tuning4) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  const ent::Tuning& t = ctx->eng->tuning;
+  tuning4[0] = t.q;
+  tuning4[1] = t.m0;
+  tuning4[2] = t.mr;
+  tuning4[3] = t.group_h;
+  return ENT_OK;
+}
+
 int ent_ctx_last_stats(ent_ctx* ctx, double* density_ms, double* total_ms, uint64_t* pairs,
                        uint64_t* tiles) {
   CAPI_TRY(check_ctx(ctx));

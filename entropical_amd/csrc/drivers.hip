@@ -119,70 +119,41 @@ ent::RedItem item(uint32_t op, std::initializer_list<uint64_t> offs, uint64_t n,
   return it;
 }
 
-}  // namespace
-
-extern "C" {
-
-int ent_mi_matrix_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
-                      int bits, int method, double* mi) {
-  DRV_TRY(check_common(ctx, X_dev, n_rows, n_cols, h));
-  if (!mi) {
-    set_error("null argument");
-    return ENT_ERR_ARG;
-  }
-  auto t0 = std::chrono::steady_clock::now();
-  const size_t C = n_cols, N = n_rows;
+// mi.cpp:37-52 job set: 2-D joints first (the larger ones), then the 1-D
+// marginals once per column (mi.cpp:43-46 recomputes p_j for every pair).
+void build_mi(size_t N, size_t C, const float* h, int bits, std::vector<ent::JobSpec>& jobs,
+              std::vector<ent::RedItem>& items) {
   const double sc = bits ? 1.0 / std::log(2.0) : 1.0;
-  std::vector<ent::JobSpec> jobs;
-  std::vector<ent::RedItem> items;
-  // 2-D jobs first (the larger ones), then the 1-D marginals computed once
-  // per column (mi.cpp:38-46 recomputes p_j for every pair).
-  const uint64_t marg0 = 0;
   uint64_t slot = C;
   for (uint32_t i = 0; i < C; ++i)
     for (uint32_t j = i + 1; j < C; ++j) {
       jobs.push_back(make_job(N, {i, j}, {h[i], h[j]}, {0, 0}, slot * N));
-      items.push_back(item(ent::kRedMI, {slot * N, (marg0 + i) * N, (marg0 + j) * N}, N, 0.0, sc));
+      items.push_back(item(ent::kRedMI, {slot * N, (uint64_t)i * N, (uint64_t)j * N}, N, 0.0, sc));
       ++slot;
     }
-  for (uint32_t c = 0; c < C; ++c) jobs.push_back(make_job(N, {c}, {h[c]}, {0}, (marg0 + c) * N));
-  std::vector<double> res;
-  DRV_TRY(run_and_reduce(ctx, X_dev, jobs, method, items, res));
-  size_t p = 0;
-  for (size_t i = 0; i < C; ++i) mi[i * C + i] = 0.0;
-  for (size_t i = 0; i < C; ++i)
-    for (size_t j = i + 1; j < C; ++j, ++p) {
-      mi[i * C + j] = res[p];
-      mi[j * C + i] = res[p];
-    }
-  ctx->driver_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  return ENT_OK;
+  for (uint32_t c = 0; c < C; ++c) jobs.push_back(make_job(N, {c}, {h[c]}, {0}, (uint64_t)c * N));
 }
 
-int ent_te_matrix_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
-                      const uint32_t* taus, size_t n_taus, int bits, int method, double* te) {
-  DRV_TRY(check_common(ctx, X_dev, n_rows, n_cols, h));
-  if (!te || !taus || n_taus == 0) {
-    set_error("null argument");
-    return ENT_ERR_ARG;
-  }
+int check_taus(size_t n_rows, const uint32_t* taus, size_t n_taus) {
   for (size_t t = 0; t < n_taus; ++t)
     if (taus[t] >= n_rows) {
       set_error("error: lag time (tau) must be smaller than the number of rows");
       return ENT_ERR_ARG;
     }
-  auto t0 = std::chrono::steady_clock::now();
-  const size_t C = n_cols, N = n_rows;
+  return ENT_OK;
+}
+
+// transs.cpp:42-74 job set for every lag (3-D jobs first), and the
+// reduction items of transs.cpp:75-113 (both directions per pair).
+void build_te(size_t N, size_t C, const float* h, const uint32_t* taus, size_t n_taus, int bits,
+              std::vector<ent::JobSpec>& jobs, std::vector<ent::RedItem>& items) {
   const double sc = bits ? 1.0 / std::log(2.0) : 1.0;
-  std::vector<ent::JobSpec> jobs;
-  std::vector<ent::RedItem> items;
   uint64_t next = 0;
   auto alloc = [&](uint64_t n) {
     uint64_t o = next;
     next += n;
     return o;
   };
-  // transs.cpp:42-74 job set.  3-D jobs first (largest), then the rest.
   std::vector<uint64_t> off_p(C), off_pij(C * C, 0);
   std::vector<std::vector<uint64_t>> off_ptau(n_taus, std::vector<uint64_t>(C));
   std::vector<std::vector<uint64_t>> off_itau(n_taus, std::vector<uint64_t>(C * C)), off_jtau = off_itau;
@@ -210,7 +181,6 @@ int ent_te_matrix_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_
     off_p[i] = alloc(N);
     jobs.push_back(make_job(N, {i}, {h[i]}, {0}, off_p[i]));
   }
-  // reduction items: per tau, per pair, both directions (transs.cpp:75-113)
   for (size_t t = 0; t < n_taus; ++t) {
     const uint64_t n = N - taus[t];
     for (uint32_t i = 0; i < C; ++i)
@@ -222,6 +192,119 @@ int ent_te_matrix_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_
                              n, hi * hj * hi, sc));
       }
   }
+}
+
+// Runs the engine's autotuner on a job set, with ctx->out as scratch output.
+int tune_jobs(ent_ctx* ctx, const float* X, const std::vector<ent::JobSpec>& jobs, int* out4, double* best_ms) {
+  ent::Engine& E = *ctx->eng;
+  uint64_t total = 0;
+  for (const auto& j : jobs) total = std::max<uint64_t>(total, j.out_off + j.n);
+  DRV_TRY(E.ensure(ctx->out, sizeof(double) * total));
+  ent::Tuning best;
+  double ms = 0.0;
+  DRV_TRY(E.autotune(X, jobs, (double*)ctx->out.p, &best, &ms));
+  if (out4) {
+    out4[0] = best.q;
+    out4[1] = best.m0;
+    out4[2] = best.mr;
+    out4[3] = best.group_h;
+  }
+  if (best_ms) *best_ms = ms;
+  return ENT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ent_mi_autotune_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
+                        int* tuning4) {
+  DRV_TRY(check_common(ctx, X_dev, n_rows, n_cols, h));
+  std::vector<ent::JobSpec> jobs;
+  std::vector<ent::RedItem> items;
+  build_mi(n_rows, n_cols, h, 0, jobs, items);
+  return tune_jobs(ctx, X_dev, jobs, tuning4, nullptr);
+}
+
+int ent_te_autotune_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
+                        const uint32_t* taus, size_t n_taus, int* tuning4) {
+  DRV_TRY(check_common(ctx, X_dev, n_rows, n_cols, h));
+  if (!taus || n_taus == 0) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  DRV_TRY(check_taus(n_rows, taus, n_taus));
+  std::vector<ent::JobSpec> jobs;
+  std::vector<ent::RedItem> items;
+  build_te(n_rows, n_cols, h, taus, n_taus, 0, jobs, items);
+  return tune_jobs(ctx, X_dev, jobs, tuning4, nullptr);
+}
+
+int ent_autotune_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs, int* tuning4,
+                     double* best_ms) {
+  if (!ctx || !ctx->eng || !X_dev || !jobs || n_jobs == 0) {
+    set_error("null argument or empty job list");
+    return ENT_ERR_ARG;
+  }
+  std::vector<ent::JobSpec> js(n_jobs);
+  uint64_t next = 0;
+  for (size_t j = 0; j < n_jobs; ++j) {
+    const ent_job& a = jobs[j];
+    if (a.n_dim < 1 || a.n_dim > 3 || a.n == 0 || a.n_rows == 0) {
+      set_error("invalid job: n_dim must be 1..3 and n, n_rows > 0");
+      return ENT_ERR_ARG;
+    }
+    js[j].d = a.n_dim;
+    js[j].n = a.n;
+    js[j].n_rows = a.n_rows;
+    js[j].out_off = next;  // tuning output is scratch: pack the jobs
+    next += a.n;
+    for (uint32_t e = 0; e < 3; ++e) {
+      js[j].off[e] = a.off[e];
+      js[j].h[e] = e < a.n_dim ? a.h[e] : 1.0f;
+    }
+  }
+  return tune_jobs(ctx, X_dev, js, tuning4, best_ms);
+}
+
+int ent_mi_matrix_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
+                      int bits, int method, double* mi) {
+  DRV_TRY(check_common(ctx, X_dev, n_rows, n_cols, h));
+  if (!mi) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  const size_t C = n_cols;
+  std::vector<ent::JobSpec> jobs;
+  std::vector<ent::RedItem> items;
+  build_mi(n_rows, n_cols, h, bits, jobs, items);
+  std::vector<double> res;
+  DRV_TRY(run_and_reduce(ctx, X_dev, jobs, method, items, res));
+  size_t p = 0;
+  for (size_t i = 0; i < C; ++i) mi[i * C + i] = 0.0;
+  for (size_t i = 0; i < C; ++i)
+    for (size_t j = i + 1; j < C; ++j, ++p) {
+      mi[i * C + j] = res[p];
+      mi[j * C + i] = res[p];
+    }
+  ctx->driver_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return ENT_OK;
+}
+
+int ent_te_matrix_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
+                      const uint32_t* taus, size_t n_taus, int bits, int method, double* te) {
+  DRV_TRY(check_common(ctx, X_dev, n_rows, n_cols, h));
+  if (!te || !taus || n_taus == 0) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  DRV_TRY(check_taus(n_rows, taus, n_taus));
+  auto t0 = std::chrono::steady_clock::now();
+  const size_t C = n_cols;
+  std::vector<ent::JobSpec> jobs;
+  std::vector<ent::RedItem> items;
+  build_te(n_rows, n_cols, h, taus, n_taus, bits, jobs, items);
   std::vector<double> res;
   DRV_TRY(run_and_reduce(ctx, X_dev, jobs, method, items, res));
   size_t p = 0;
@@ -269,6 +352,44 @@ int ent_batch_entropy_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs,
   DRV_TRY(run_and_reduce(ctx, X_dev, js, method, items, res));
   for (size_t j = 0; j < n_jobs; ++j) H[j] = -res[j] / (double)jobs[j].n;
   ctx->driver_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return ENT_OK;
+}
+
+int ent_batch_loo_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs, double floor_rel,
+                      int method, double* loglik) {
+  if (!ctx || !ctx->eng || !X_dev || !jobs || !loglik) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  if (!(floor_rel > 0.0)) {
+    set_error("floor_rel must be > 0");
+    return ENT_ERR_ARG;
+  }
+  std::vector<ent::JobSpec> js(n_jobs);
+  std::vector<ent::RedItem> items(n_jobs);
+  uint64_t next = 0;
+  for (size_t j = 0; j < n_jobs; ++j) {
+    const ent_job& a = jobs[j];
+    if (a.n_dim < 1 || a.n_dim > 3 || a.n < 2) {
+      set_error("invalid job: n_dim must be 1..3 and n >= 2");
+      return ENT_ERR_ARG;
+    }
+    js[j].d = a.n_dim;
+    js[j].n = a.n;
+    js[j].n_rows = a.n;  // p = S * K0 / n, so n p - K0 = (S - 1) K0
+    js[j].out_off = next;
+    double k0 = 1.0;
+    for (uint32_t e = 0; e < 3; ++e) {
+      js[j].off[e] = a.off[e];
+      js[j].h[e] = e < a.n_dim ? a.h[e] : 1.0f;
+      if (e < a.n_dim) k0 *= 0.75 / (double)a.h[e];
+    }
+    items[j] = item(ent::kRedLoo, {next}, a.n, k0, floor_rel * k0 / (double)(a.n - 1));
+    next += a.n;
+  }
+  std::vector<double> res;
+  DRV_TRY(run_and_reduce(ctx, X_dev, js, method, items, res));
+  for (size_t j = 0; j < n_jobs; ++j) loglik[j] = res[j];
   return ENT_OK;
 }
 

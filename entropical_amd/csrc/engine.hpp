@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 #include "kde_engine.hpp"
@@ -88,6 +89,14 @@ class Engine {
   Tuning tuning;
   RunStats last;
 
+  // Autotuner: coordinate descent over (q, m0, mr, group_h) on a subsample of
+  // the jobs, timing whole runs (planning round trip, sort, tiles, density
+  // launch) with HIP events.  Sets `tuning` to the best and caches it under a
+  // signature of the workload (in memory, and in the file named by
+  // ENT_TUNE_CACHE when set).
+  int autotune(const float* X, const std::vector<JobSpec>& jobs, double* dens, Tuning* best,
+               double* best_ms);
+
   // grow-only device workspace
   int ensure(DevBuf& b, size_t bytes);
   int ensure_host(size_t bytes);
@@ -108,6 +117,8 @@ class Engine {
 
   GraphCache pre_graphs_, dens_graphs_;
   uint64_t generation_ = 0;  // bumped on every workspace reallocation
+  std::map<std::string, Tuning> tune_cache_;
+  bool tune_cache_loaded_ = false;
 
  public:
   bool use_graph = true;

@@ -65,10 +65,10 @@ struct DTile {
 };
 
 struct Tuning {
-  int q = 4;            // queries per lane (tile = 64*q queries)
-  int m0 = 4;           // dim-0 boxes per bandwidth
-  int mr = 2;           // row-dim boxes per bandwidth
-  int group_h = 2;      // tile extent bound along dim 0, in bandwidths
+  int q = 2;            // maximum queries per lane (tile = 64*q queries)
+  int m0 = 8;           // dim-0 boxes per bandwidth
+  int mr = 1;           // row-dim boxes per bandwidth
+  int group_h = 4;      // tile extent bound along dim 0, in bandwidths
 };
 
 }  // namespace ent

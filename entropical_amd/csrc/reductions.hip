@@ -38,6 +38,11 @@ __device__ __forceinline__ double term(const RedItem& it, uint64_t k, const doub
       const double r = D[it.off[1] + k] * inv0(a * it.pref);
       return r == 0.0 ? 0.0 : a * it.pref * it.pref * log(r) * it.scale;
     }
+    case kRedLoo: {  // p = density incl. self pair, divided by n (n_rows == n)
+      const double nd = (double)it.n;
+      const double loo = (nd * a - it.pref) / (nd - 1.0);
+      return log(loo > it.scale ? loo : it.scale);
+    }
     default:
       return 0.0;
   }

@@ -10,6 +10,9 @@ enum RedOp : uint32_t {
   kRedLogMean = 1,  // sum ln p                                 (H = -mean ln p)
   kRedTE = 2,       // sum p3 pref log(p3 p1 inv(p2) inv(ptau))  (transs.cpp:77-100)
   kRedAutoEnt = 3,  // sum p2 h h log(p1 inv(p2 h)), pref = h   (autoent.cpp:52-56)
+  kRedLoo = 4,      // sum ln max(p_{-i}, floor), p_{-i} = (n p - K0) / (n - 1):
+                    // leave-one-out log-likelihood (p includes the self pair
+                    // K0 = prod 0.75/h); pref = K0, scale = floor density
 };
 
 // off[0..3] index density arrays in one device buffer D (doubles).

@@ -65,8 +65,13 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
-    def set_tuning(self, q: int = 4, m0: int = 4, mr: int = 2, group_h: int = 2) -> None:
+    def set_tuning(self, q: int = 2, m0: int = 8, mr: int = 1, group_h: int = 4) -> None:
         check(load().ent_ctx_set_tuning(self.handle, q, m0, mr, group_h))
+
+    def get_tuning(self) -> tuple:
+        t = (C.c_int * 4)()
+        check(load().ent_ctx_get_tuning(self.handle, t))
+        return tuple(int(v) for v in t)
 
     def set_graph(self, enable: bool) -> None:
         check(load().ent_ctx_set_graph(self.handle, 1 if enable else 0))
@@ -305,3 +310,31 @@ def batch_entropy(ctx: Context, x_dev_ptr: int, specs: Sequence[dict], method: s
     check(load().ent_batch_entropy_dev(ctx.handle, C.c_void_p(x_dev_ptr), arr, len(specs),
                                        _METHODS[method], _ptr(out)))
     return out
+
+
+def mi_autotune(ctx: Context, x: DeviceArray, h: Sequence[float]) -> tuple:
+    """Tune the engine for the MI job set of ``x`` (kept on ``ctx``)."""
+    n_cols, n_rows = x.shape
+    hh = _f32(h)
+    t = (C.c_int * 4)()
+    check(load().ent_mi_autotune_dev(ctx.handle, C.c_void_p(x.ptr), n_rows, n_cols, _ptr(hh), t))
+    return tuple(int(v) for v in t)
+
+
+def te_autotune(ctx: Context, x: DeviceArray, h: Sequence[float], taus: Sequence[int]) -> tuple:
+    """Tune the engine for the transfer-entropy job set (kept on ``ctx``)."""
+    n_cols, n_rows = x.shape
+    hh, tt = _f32(h), _u32(taus)
+    t = (C.c_int * 4)()
+    check(load().ent_te_autotune_dev(ctx.handle, C.c_void_p(x.ptr), n_rows, n_cols, _ptr(hh), _ptr(tt),
+                                     tt.size, t))
+    return tuple(int(v) for v in t)
+
+
+def batch_autotune(ctx: Context, x_dev_ptr: int, specs: Sequence[dict]) -> tuple:
+    """Tune the engine for an arbitrary batch of jobs (kept on ``ctx``)."""
+    arr = make_jobs(specs)
+    t = (C.c_int * 4)()
+    ms = C.c_double()
+    check(load().ent_autotune_dev(ctx.handle, C.c_void_p(x_dev_ptr), arr, len(specs), t, C.byref(ms)))
+    return tuple(int(v) for v in t)

@@ -1,0 +1,326 @@
+"""Learned bandwidth selection.
+
+Bandwidth family: per dimension e, ``h_e = r * thumb_e`` with the reference's
+rule of thumb ``thumb_e = N^(-1/7) * sigma_e`` (Hestimate::Thumb::Kernel,
+hestimate.cpp:19-30).  A small MLP (24 features -> 128 -> 128 -> 128 -> 1,
+tanh, BF16 weights, FP32 accumulation; trained and evaluated by the HIP
+kernels in mlp.hip) predicts ``log2 r``.  Labels are the leave-one-out
+log-likelihood argmax over 32 candidate multipliers, computed by the GPU KDE
+(``ent_batch_loo_dev``).  The rule of thumb (and AMISE through
+``epa_convolution``) remain the fallback and the yardstick.
+
+Training data: seeded synthetic samples with d in 1..3,
+N log-uniform in [1000, 50000], 1-5 Gaussian or Student-t components with
+random covariances.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import time
+from pathlib import Path
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, load
+from .kde import Context, _ptr, make_jobs
+
+N_FEATURES = 24
+HIDDEN = 128
+HIDDEN_LAYERS = 3
+R_GRID = 2.0 ** np.linspace(-3.0, 2.0, 32)  # candidate multipliers of the rule of thumb
+FLOOR_REL = 1e-3                            # LOO floor: 1/1000 of one coincident neighbour
+WEIGHTS = Path(__file__).resolve().parent / "learned_h_weights.npz"
+
+
+# ----------------------------------------------------------------- data
+def synth_dataset(rng: np.random.Generator, n_min: int = 1000, n_max: int = 50000) -> np.ndarray:
+    """One training sample: (d, N) float32 points from a random mixture."""
+    d = int(rng.integers(1, 4))
+    n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+    k = int(rng.integers(1, 6))
+    w = rng.dirichlet(np.ones(k))
+    comp = rng.choice(k, size=n, p=w)
+    x = np.empty((n, d))
+    for c in range(k):
+        m = comp == c
+        cnt = int(m.sum())
+        if cnt == 0:
+            continue
+        mean = rng.normal(0.0, 2.0, d)
+        q, _ = np.linalg.qr(rng.normal(size=(d, d)))
+        L = q * rng.uniform(0.2, 1.5, d)
+        z = rng.standard_normal((cnt, d))
+        if rng.random() < 0.5:  # Student-t component
+            nu = float(rng.integers(3, 11))
+            z *= np.sqrt(nu / rng.chisquare(nu, size=(cnt, 1)))
+        x[m] = mean + z @ L.T
+    return np.ascontiguousarray(x.T.astype(np.float32))
+
+
+def thumb(x: np.ndarray) -> np.ndarray:
+    """Rule-of-thumb bandwidth per dimension, N^(-1/7) * population sd
+    (hestimate.cpp:19-30, variance in FP64)."""
+    x = np.asarray(x, dtype=np.float64)
+    n = x.shape[1]
+    return (n ** (-1.0 / 7.0)) * x.std(axis=1)
+
+
+_QS = np.array([0.01, 0.05, 0.10, 0.25, 0.50, 0.75, 0.90, 0.95, 0.99])
+
+
+def features(x: np.ndarray) -> np.ndarray:
+    """24 sample statistics of a (d, N) sample (scale-free: per-dim z-scores)."""
+    x = np.asarray(x, dtype=np.float64)
+    d, n = x.shape
+    mu = x.mean(axis=1, keepdims=True)
+    sd = x.std(axis=1, keepdims=True)
+    sd[sd == 0] = 1.0
+    z = (x - mu) / sd
+    q = np.quantile(z, _QS, axis=1)               # (9, d)
+    skew = (z ** 3).mean(axis=1)
+    kurt = (z ** 4).mean(axis=1) - 3.0
+    iqr = q[5] - q[3]
+    f = np.zeros(N_FEATURES)
+    f[0] = np.log10(n)
+    f[1] = d
+    f[2:11] = q.mean(axis=1)
+    f[11] = np.abs(skew).mean()
+    f[12] = np.clip(kurt, -2.0, 20.0).mean() / 10.0
+    f[13] = iqr.mean()
+    f[14] = (q[8] - q[0]).mean()
+    f[15] = ((skew ** 2 + 1.0) / (kurt + 3.0)).mean()
+    if d > 1:
+        cr = np.corrcoef(z)
+        off = np.abs(cr[~np.eye(d, dtype=bool)])
+        f[16] = off.mean()
+        f[17] = off.max()
+        f[18] = max(np.log(max(np.linalg.det(cr), 1e-12)), -10.0) / 10.0
+    f[19] = np.minimum(1.0, iqr / 1.349).mean()
+    f[20] = (np.abs(z) > 3.0).mean() * 10.0
+    f[21] = np.abs(z).mean(axis=1).mean()
+    f[22] = np.log2(n ** (-1.0 / (d + 4)) / n ** (-1.0 / 7.0))
+    modes = []
+    for e in range(d):
+        hcount, _ = np.histogram(z[e], bins=32, range=(-4, 4))
+        hs = np.convolve(hcount, np.ones(3) / 3.0, mode="same")
+        modes.append(np.sum((hs[1:-1] > hs[:-2]) & (hs[1:-1] >= hs[2:]) & (hs[1:-1] > 0.01 * hs.max())))
+    f[23] = np.mean(modes) / 5.0
+    return f.astype(np.float32)
+
+
+# --------------------------------------------------------------- labels
+def loo_grid(ctx: Context, datasets: Sequence[np.ndarray], r_grid=R_GRID, floor_rel: float = FLOOR_REL,
+             max_points: int = 400_000_000) -> np.ndarray:
+    """LOO log-likelihood (GPU KDE) of every dataset at h = r * thumb for each
+    r in r_grid.  Returns (len(datasets), len(r_grid)) float64."""
+    out = np.empty((len(datasets), len(r_grid)))
+    i = 0
+    while i < len(datasets):
+        # group datasets so that points x candidates stay under max_points
+        j, pts = i, 0
+        while j < len(datasets) and (pts == 0 or pts + datasets[j].shape[1] * len(r_grid) <= max_points):
+            pts += datasets[j].shape[1] * len(r_grid)
+            j += 1
+        group = datasets[i:j]
+        flat = np.concatenate([g.reshape(-1) for g in group]).astype(np.float32)
+        dptr = ctx.alloc(flat.nbytes)
+        try:
+            ctx.h2d(dptr, flat)
+            specs, off = [], 0
+            for g in group:
+                d, n = g.shape
+                th = thumb(g)
+                th[th <= 0] = 1.0
+                for r in r_grid:
+                    specs.append({"n_dim": d, "n": n, "n_rows": n, "off": [off + e * n for e in range(d)],
+                                  "h": [float(r * th[e]) for e in range(d)]})
+                off += d * n
+            arr = make_jobs(specs)
+            res = np.empty(len(specs))
+            check(load().ent_batch_loo_dev(ctx.handle, C.c_void_p(dptr), arr, len(specs), float(floor_rel),
+                                           _lib.METHOD_CELL, _ptr(res)))
+        finally:
+            ctx.free(dptr)
+        out[i:j] = res.reshape(len(group), len(r_grid))
+        i = j
+    return out
+
+
+def label_from_grid(ll: np.ndarray, r_grid=R_GRID) -> float:
+    """log2 of the LOO-optimal multiplier, refined by a parabola through the
+    best grid point and its neighbours (in log2 r)."""
+    lr = np.log2(r_grid)
+    k = int(np.argmax(ll))
+    if 0 < k < len(ll) - 1:
+        y0, y1, y2 = ll[k - 1], ll[k], ll[k + 1]
+        den = y0 - 2 * y1 + y2
+        if den < 0:
+            step = lr[1] - lr[0]
+            return float(lr[k] + 0.5 * step * (y0 - y2) / den)
+    return float(lr[k])
+
+
+# ---------------------------------------------------------------- model
+class BandwidthModel:
+    """The MLP on the device (ent_mlp) plus its input normalisation."""
+
+    def __init__(self, ctx: Context, seed: int = 0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(load().ent_mlp_create(ctx.handle, N_FEATURES, HIDDEN, HIDDEN_LAYERS, seed, C.byref(h)))
+        self._h = h
+        n = C.c_size_t()
+        check(load().ent_mlp_num_params(h, C.byref(n)))
+        self.n_params = n.value
+        self.mean = np.zeros(N_FEATURES, np.float32)
+        self.std = np.ones(N_FEATURES, np.float32)
+
+    def close(self) -> None:
+        if self._h is not None:
+            load().ent_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # parameters ---------------------------------------------------------
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.n_params, np.float32)
+        check(load().ent_mlp_get_params(self.ctx.handle, self._h, _ptr(p)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        check(load().ent_mlp_set_params(self.ctx.handle, self._h, _ptr(p)))
+
+    def save(self, path: Path, meta: Optional[dict] = None) -> None:
+        np.savez_compressed(path, params=self.get_params(), mean=self.mean, std=self.std,
+                            meta=np.array(json.dumps(meta or {})))
+
+    @classmethod
+    def load(cls, ctx: Context, path: Path = WEIGHTS) -> "BandwidthModel":
+        z = np.load(path, allow_pickle=False)
+        m = cls(ctx)
+        m.set_params(z["params"])
+        m.mean = z["mean"].astype(np.float32)
+        m.std = z["std"].astype(np.float32)
+        return m
+
+    # compute -----------------------------------------------------------
+    def _norm(self, F: np.ndarray) -> np.ndarray:
+        return np.ascontiguousarray(((np.asarray(F, np.float32) - self.mean) / self.std).astype(np.float32))
+
+    def predict_raw(self, Xn: np.ndarray) -> np.ndarray:
+        Xn = np.ascontiguousarray(Xn, dtype=np.float32)
+        y = np.empty(Xn.shape[0], np.float32)
+        check(load().ent_mlp_predict(self.ctx.handle, self._h, _ptr(Xn), Xn.shape[0], _ptr(y)))
+        return y
+
+    def gradient_raw(self, Xn: np.ndarray, t: np.ndarray):
+        Xn = np.ascontiguousarray(Xn, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_float()
+        check(load().ent_mlp_gradient(self.ctx.handle, self._h, _ptr(Xn), _ptr(t), Xn.shape[0], _ptr(g),
+                                      C.byref(loss)))
+        return g, loss.value
+
+    def train_raw(self, Xn: np.ndarray, t: np.ndarray, epochs: int, batch: int, lr: float, seed: int = 0):
+        Xn = np.ascontiguousarray(Xn, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        n = Xn.shape[0]
+        rng = np.random.default_rng(seed)
+        order = np.concatenate([rng.permutation(n) for _ in range(epochs)]).astype(np.uint32)
+        losses = np.empty(epochs, np.float32)
+        check(load().ent_mlp_train(self.ctx.handle, self._h, _ptr(Xn), _ptr(t), n, _ptr(order), epochs, batch,
+                                   float(lr), _ptr(losses)))
+        return losses
+
+    def fit(self, F: np.ndarray, t: np.ndarray, epochs: int = 200, batch: int = 256, lr: float = 1e-3,
+            seed: int = 0) -> np.ndarray:
+        F = np.asarray(F, np.float32)
+        self.mean = F.mean(axis=0)
+        self.std = F.std(axis=0)
+        self.std[self.std < 1e-6] = 1.0
+        return self.train_raw(self._norm(F), t, epochs, batch, lr, seed)
+
+    def predict_log2r(self, F: np.ndarray) -> np.ndarray:
+        lr = self.predict_raw(self._norm(np.atleast_2d(F)))
+        lo, hi = np.log2(R_GRID[0]), np.log2(R_GRID[-1])
+        return np.clip(lr, lo, hi)
+
+    def bandwidth(self, x: np.ndarray) -> np.ndarray:
+        """Per-dimension bandwidth for a (d, N) sample."""
+        return thumb(x) * 2.0 ** float(self.predict_log2r(features(x))[0])
+
+
+# -------------------------------------------------------------- training
+def make_training_set(ctx: Context, n_datasets: int, seed: int, log=print):
+    rng = np.random.default_rng(seed)
+    F = np.empty((n_datasets, N_FEATURES), np.float32)
+    Y = np.empty(n_datasets, np.float32)
+    LL = np.empty((n_datasets, len(R_GRID)))
+    dims = np.empty(n_datasets, np.int32)
+    chunk = 500
+    t0 = time.time()
+    for s in range(0, n_datasets, chunk):
+        ds = [synth_dataset(rng) for _ in range(min(chunk, n_datasets - s))]
+        ll = loo_grid(ctx, ds)
+        for k, (x, row) in enumerate(zip(ds, ll)):
+            F[s + k] = features(x)
+            Y[s + k] = label_from_grid(row)
+            LL[s + k] = row
+            dims[s + k] = x.shape[0]
+        log(f"[learned_h] labelled {s + len(ds)}/{n_datasets} datasets ({time.time() - t0:.1f} s)")
+    return F, Y, LL, dims
+
+
+def main(argv=None) -> int:
+    import argparse
+
+    ap = argparse.ArgumentParser(description="train the learned bandwidth model on the GPU")
+    ap.add_argument("--datasets", type=int, default=20000)
+    ap.add_argument("--holdout", type=int, default=1000)
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--seed", type=int, default=0xE17A0005)
+    ap.add_argument("--out", type=Path, default=WEIGHTS)
+    ap.add_argument("--report", type=Path, default=None)
+    a = ap.parse_args(argv)
+    ctx = Context(0)
+    F, Y, LL, dims = make_training_set(ctx, a.datasets + a.holdout, a.seed)
+    ntr = a.datasets
+    model = BandwidthModel(ctx, seed=1)
+    t0 = time.time()
+    losses = model.fit(F[:ntr], Y[:ntr], epochs=a.epochs)
+    train_s = time.time() - t0
+    pred = model.predict_log2r(F[ntr:])
+    lr = np.log2(R_GRID)
+    ll_at = lambda row, l2r: np.interp(l2r, lr, row)
+    gain_mlp = np.array([ll_at(LL[ntr + k], pred[k]) for k in range(a.holdout)])
+    gain_thumb = np.array([ll_at(LL[ntr + k], 0.0) for k in range(a.holdout)])
+    best = LL[ntr:].max(axis=1)
+    rep = {
+        "datasets": a.datasets, "holdout": a.holdout, "epochs": a.epochs, "train_s": train_s,
+        "final_train_mse": float(losses[-1]), "first_train_mse": float(losses[0]),
+        "holdout_mae_log2r_mlp": float(np.mean(np.abs(pred - Y[ntr:]))),
+        "holdout_mae_log2r_thumb": float(np.mean(np.abs(Y[ntr:]))),
+        "holdout_mean_loo_gap_mlp": float(np.mean(best - gain_mlp)),
+        "holdout_mean_loo_gap_thumb": float(np.mean(best - gain_thumb)),
+        "holdout_frac_mlp_better": float(np.mean(gain_mlp > gain_thumb)),
+    }
+    model.save(a.out, meta=rep)
+    print(json.dumps(rep), flush=True)
+    if a.report:
+        a.report.write_text(json.dumps(rep, indent=1))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

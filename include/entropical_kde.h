@@ -167,6 +167,31 @@ int ent_te_matrix_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_
 int ent_batch_entropy_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs,
                           int method, double* H);
 
+/* Leave-one-out log-likelihood sum_k ln max(p_{-k}(x_k), floor) of every job
+ * (bandwidth labels of the learned model; no reference counterpart).
+ * p_{-k} excludes the self pair; floor = floor_rel * prod(0.75/h) / (n-1),
+ * i.e. floor_rel of one coincident neighbour.  jobs[j].n_rows is ignored. */
+int ent_batch_loo_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs, double floor_rel,
+                      int method, double* loglik);
+
+/* Tile-shape autotuner (no reference counterpart; the reference fixes its
+ * tile at compile time, densities_cuda.cu:14).  Times candidate settings of
+ * (q, m0, mr, group_h) on a subsample of the job set by coordinate descent,
+ * keeps the fastest on the context (and returns it in tuning4 when non-NULL),
+ * and caches it per workload signature (file named by ENT_TUNE_CACHE, if set).
+ * The MI/TE variants tune on the job sets of ent_mi_matrix_dev /
+ * ent_te_matrix_dev; the generic one on an arbitrary batch (best_ms: time of
+ * the winning setting on the subsample, -1 on a cache hit). */
+int ent_mi_autotune_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
+                        int* tuning4);
+int ent_te_autotune_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
+                        const uint32_t* taus, size_t n_taus, int* tuning4);
+int ent_autotune_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs, int* tuning4,
+                     double* best_ms);
+
+/* Current engine tuning (q, m0, mr, group_h). */
+int ent_ctx_get_tuning(ent_ctx* ctx, int* tuning4);
+
 /* Wall time (ms) of the last driver call on this context. */
 int ent_ctx_last_driver_ms(ent_ctx* ctx, double* ms);
 

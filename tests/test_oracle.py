@@ -316,3 +316,31 @@ def test_loo_closed_form(orc):
     u = float(np.float32(delta)) / h
     k = 0.75 * (1 - u * u) / h
     assert ll[0] == pytest.approx(2 * math.log(k), rel=1e-12)
+
+
+# ---------------------------------------------------------- MLP oracle
+def test_mlp_oracle_gradient_matches_finite_differences():
+    from oracle import mlp_oracle as M
+
+    r = rng(20)
+    dims = [5, 7, 6, 1]
+    n = sum(a * b + b for a, b in zip(dims[:-1], dims[1:]))
+    p = r.normal(0, 0.5, n)
+    X = r.normal(size=(9, 5))
+    t = r.normal(size=9)
+    _, g = M.loss_and_grad(p, dims, X, t, bf16=False)
+    eps = 1e-6
+    for k in r.choice(n, 25, replace=False):
+        e = np.zeros(n)
+        e[k] = eps
+        lp, _ = M.loss_and_grad(p + e, dims, X, t, bf16=False)
+        lm, _ = M.loss_and_grad(p - e, dims, X, t, bf16=False)
+        assert g[k] == pytest.approx((lp - lm) / (2 * eps), rel=1e-6, abs=1e-9)
+
+
+def test_bf16_rounding_known_values():
+    from oracle import mlp_oracle as M
+
+    x = np.array([1.0, 1.00390625, 1.005859375, -3.14159265, 65504.0], np.float32)
+    # 1.00390625 = 1 + 2^-8 is a tie between 1 and 1+2^-7 -> even (1.0)
+    np.testing.assert_array_equal(M.bf16_round(x), [1.0, 1.0, 1.0078125, -3.140625, 65536.0])
