@@ -1,20 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark of the batched Epanechnikov KDE hot path on one MI355X.
 
-Workload (default): the all-pairs mutual-information matrix of 64 columns x
+Default workload: the all-pairs mutual-information matrix of 64 columns x
 200,000 frames (seeded 64x64 N(0,1) mixing of independent 2-component
 Gaussian mixtures, z-scored per column, float32), bandwidth 0.15 per dim:
 64 1-D + 2016 2-D densities in one batched persistent launch, MI reduced on
 the device in FP64.  A step is one full MI matrix from device-resident
-coordinates (binning, sort, density launch, reductions, 64x64 result to host).
+coordinates (planning, binning, sort, tiles, density launch, reductions,
+64x64 result to host).
 
-Printed JSON line: value = executed pair evaluations per second (Gpair-evals/s)
-over the whole step; the dense-equivalent rate, the plain per-pair GPU path
-(the reference's 4096-call driver order, extrapolated from timed calls) and
-the host OpenMP baseline (oracle restatement of densities_omp.cpp on a timed
-subset, extrapolated) are reported beside it.  Roofline: FP32 VALU, achieved
-= algorithmic FLOPs (4d per executed pair) / duration of the persistent
-density kernel measured with HIP events.
+Other workloads (--workload): te (32 coupled AR(1) series x 100,000 frames,
+h=0.2, lags 1,2,5,10), uneven (500 trajectories of 2,000-500,000 frames,
+d 1-3), entropy1d (N=100,000 three-component mixture, h=0.1).
+
+Printed JSON line: value = executed pair evaluations per second
+(Gpair-evals/s) over the whole step; the dense-equivalent rate, the plain
+per-pair GPU path (the reference's per-call driver order, extrapolated from
+timed calls) and the host OpenMP baseline (oracle restatement of
+densities_omp.cpp on timed calls, extrapolated) are reported beside it.
+Roofline: FP32 VALU; achieved = algorithmic FLOPs (4d per executed pair) /
+duration of the persistent density kernel, measured with HIP events on the
+engine's stream.  The tile shape is chosen by the engine's autotuner during
+warmup (untimed).
 """
 from __future__ import annotations
 
@@ -51,31 +58,49 @@ def time_call(fn, reps: int = 3) -> float:
     return float(np.median(ts))
 
 
-def cpu_baseline_mi(X: np.ndarray, h: float, threads: int) -> dict:
-    """Host OpenMP path (the oracle's restatement of densities_omp.cpp: same
-    sort, O(N^2) linear window scans, FP32 window sums, same schedules) timed
-    on one 1-D and one 2-D density of the workload, extrapolated to the 4096
-    calls of mi.cpp's driver loop (2080 1-D + 2016 2-D)."""
-    import oracle
-
-    oracle.set_threads(threads)
-    n = X.shape[1]
-    t1 = time_call(lambda: oracle.density_omp_packed(X[0:1], n, [h]), reps=1)
-    log(f"cpu 1-D density {t1:.2f} s")
-    t2 = time_call(lambda: oracle.density_omp_packed(X[0:2], n, [h, h]), reps=1)
-    log(f"cpu 2-D density {t2:.2f} s")
-    total = 2080 * t1 + 2016 * t2
+def roofline(pairs_by_d, kern_ms: float) -> dict:
+    flops = sum(p * f for p, f in zip(pairs_by_d, FLOPS_PER_PAIR))
+    achieved = flops / (kern_ms * 1e-3) / 1e12
     return {
-        "value": total,
-        "unit": "s per MI matrix (extrapolated)",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"one 1-D and one 2-D density at N={n} ({t1:.2f} s, {t2:.2f} s), scaled by mi.cpp's 2080+2016 calls",
+        "bound": "valu",
+        "kernel": "k_density (persistent)",
+        "achieved": achieved,
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / PEAK_FP32_TFLOPS,
+        "traffic": None,
+        "kernel_ms": kern_ms,
+        "flops_per_pair": "4d (d=1: 4, d=2: 8, d=3: 12)",
     }
 
 
+def timed_steps(ctx, step, steps: int, warmup: int):
+    for _ in range(warmup):
+        step()
+    ctx.sync()
+    dens_ms, util = [], []
+    st = None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+        st = ctx.last_stats()
+        dens_ms.append(st["density_ms"])
+        util.append(sum(st["pairs_by_d"]) / max(1, st["lane_pairs"]))
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    return elapsed / steps, float(np.mean(dens_ms)), st, float(np.mean(util))
+
+
+def cpu_omp_time(packed: np.ndarray, n: int, h, threads: int) -> float:
+    import oracle
+
+    oracle.set_threads(threads)
+    return time_call(lambda: oracle.density_omp_packed(packed, n, h), reps=1)
+
+
+# ------------------------------------------------------------------ MI
 def run_mi(args) -> dict:
-    from entropical_amd import Context, DeviceArray, combined_densities, datasets, mi_matrix
+    from entropical_amd import Context, DeviceArray, combined_densities, datasets, mi_autotune, mi_matrix
 
     n_cols, n_rows, h = 64, 200_000, 0.15
     log("generating 64 x 200000 mixed columns")
@@ -83,62 +108,32 @@ def run_mi(args) -> dict:
     hs = [h] * n_cols
     ctx = Context(0)
     dx = DeviceArray(ctx, X)
-    log("warmup")
-    for _ in range(args.warmup):
-        mi_matrix(ctx, dx, hs)
-    ctx.sync()
-    log(f"timing {args.steps} steps")
-    dens_ms, pairs_by_d = [], None
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        mi = mi_matrix(ctx, dx, hs)
-        st = ctx.last_stats()
-        dens_ms.append(st["density_ms"])
-        pairs_by_d = st["pairs_by_d"]
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
-    ms_step = 1e3 * elapsed / args.steps
-    pairs = sum(pairs_by_d)
-    flops = sum(p * f for p, f in zip(pairs_by_d, FLOPS_PER_PAIR))
-    kern_ms = float(np.mean(dens_ms))
-    achieved = flops / (kern_ms * 1e-3) / 1e12
-    dense_pairs = 2080 * n_rows * n_rows
-    log(f"MI matrix {ms_step:.1f} ms/step, density kernel {kern_ms:.1f} ms, {pairs:.3e} pairs")
+    tun = mi_autotune(ctx, dx, hs) if not args.no_tune else ctx.get_tuning()
+    log(f"tuning (q, m0, mr, group_h) = {tun}")
+    sec, kern_ms, st, util = timed_steps(ctx, lambda: mi_matrix(ctx, dx, hs), args.steps, args.warmup)
+    mi = mi_matrix(ctx, dx, hs)
     assert np.all(np.isfinite(mi)) and np.allclose(mi, mi.T)
-
+    pairs = sum(st["pairs_by_d"])
+    log(f"MI matrix {1e3 * sec:.1f} ms/step, density kernel {kern_ms:.1f} ms, {pairs:.3e} pairs")
     res = {
         "metric": "Gpair-evals/s (batched 64-column MI matrix, 200k frames)",
-        "value": pairs / (elapsed / args.steps) / 1e9,
+        "value": pairs / sec / 1e9,
         "unit": "Gpair-evals/s",
-        "ms_per_step": ms_step,
-        "dtype": "f32",
+        "ms_per_step": 1e3 * sec,
         "config": {
             "workload": "mi_matrix_64col_200000frames_h0.15",
             "jobs": "64 1-D + 2016 2-D densities, one persistent launch",
-            "n_cols": n_cols,
-            "n_rows": n_rows,
-            "bandwidth": h,
-            "method": "cell list",
+            "n_cols": n_cols, "n_rows": n_rows, "bandwidth": h, "method": "cell list",
+            "tuning_q_m0_mr_group": list(tun),
         },
         "pairs_executed": pairs,
-        "pairs_by_dim": pairs_by_d,
-        "dense_equiv_gpair_per_s": dense_pairs / (elapsed / args.steps) / 1e9,
-        "roofline": {
-            "bound": "valu",
-            "kernel": "k_density (persistent)",
-            "achieved": achieved,
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
-            "kernel_ms": kern_ms,
-            "flops_per_pair": "4d (d=1: 4, d=2: 8)",
-        },
+        "pairs_by_dim": st["pairs_by_d"],
+        "lane_utilisation": util,
+        "dense_equiv_gpair_per_s": 2080 * n_rows * n_rows / sec / 1e9,
+        "roofline": roofline(st["pairs_by_d"], kern_ms),
     }
     if not args.no_plain:
-        # plain per-pair GPU path: the reference's driver order, each call
-        # with its H2D, density launch and D2H (4096 calls for 64 columns)
-        log("timing plain per-pair path (one call each, extrapolated)")
+        log("timing plain per-pair path (one call each, extrapolated to mi.cpp's 4096 calls)")
         plain = {}
         for method in ("brute", "cell"):
             combined_densities(X, n_rows, [0], [h], method=method, ctx=ctx)
@@ -146,16 +141,150 @@ def run_mi(args) -> dict:
             t2 = time_call(lambda: combined_densities(X, n_rows, [0, 1], [h, h], method=method, ctx=ctx))
             plain[method] = {"t_1d_ms": 1e3 * t1, "t_2d_ms": 1e3 * t2, "mi_matrix_ms": 1e3 * (2080 * t1 + 2016 * t2)}
         res["plain_path"] = plain
-        res["speedup_vs_plain_brute"] = plain["brute"]["mi_matrix_ms"] / ms_step
-        res["speedup_vs_plain_cell"] = plain["cell"]["mi_matrix_ms"] / ms_step
+        res["speedup_vs_plain_brute"] = plain["brute"]["mi_matrix_ms"] / (1e3 * sec)
+        res["speedup_vs_plain_cell"] = plain["cell"]["mi_matrix_ms"] / (1e3 * sec)
     dx.free()
     ctx.close()
     if not args.no_cpu_baseline:
-        log("cpu baseline")
-        cb = cpu_baseline_mi(X, h, args.cpu_threads or host_cores())
-        res["cpu_baseline"] = cb
-        res["speedup_vs_cpu"] = cb["value"] * 1e3 / ms_step
+        th = args.cpu_threads or host_cores()
+        t1 = cpu_omp_time(X[0:1], n_rows, [h], th)
+        log(f"cpu 1-D density {t1:.2f} s")
+        t2 = cpu_omp_time(X[0:2], n_rows, [h, h], th)
+        log(f"cpu 2-D density {t2:.2f} s")
+        total = 2080 * t1 + 2016 * t2
+        res["cpu_baseline"] = {
+            "value": total, "unit": "s per MI matrix (extrapolated)", "cores": th, "kind": "port",
+            "sample": f"one 1-D and one 2-D density at N={n_rows} ({t1:.2f} s, {t2:.2f} s) with the oracle's "
+                      "restatement of densities_omp.cpp, scaled by mi.cpp's 2080 + 2016 calls",
+        }
+        res["speedup_vs_cpu"] = total / sec
     return res
+
+
+# ------------------------------------------------------------------ TE
+def run_te(args) -> dict:
+    from entropical_amd import Context, DeviceArray, combined_densities, datasets, te_autotune, te_matrix
+
+    n_cols, n_rows, h, taus = 32, 100_000, 0.2, [1, 2, 5, 10]
+    log("generating 32 coupled AR(1) series x 100000")
+    X, _, _ = datasets.coupled_ar1(n_cols, n_rows)
+    hs = [h] * n_cols
+    ctx = Context(0)
+    dx = DeviceArray(ctx, X)
+    tun = te_autotune(ctx, dx, hs, taus) if not args.no_tune else ctx.get_tuning()
+    log(f"tuning (q, m0, mr, group_h) = {tun}")
+    sec, kern_ms, st, util = timed_steps(ctx, lambda: te_matrix(ctx, dx, hs, taus), args.steps, args.warmup)
+    pairs = sum(st["pairs_by_d"])
+    log(f"TE (4 lags) {1e3 * sec:.1f} ms/step, density kernel {kern_ms:.1f} ms, {pairs:.3e} pairs")
+    n_pairs = n_cols * (n_cols - 1) // 2
+    jobs_3d, jobs_2d, jobs_1d = 2 * n_pairs * len(taus), n_pairs + n_cols * len(taus), n_cols
+    dense = (jobs_3d + jobs_2d + jobs_1d) * float(n_rows) ** 2
+    res = {
+        "metric": "Gpair-evals/s (transfer entropy, 32 series x 100k frames, 4 lags)",
+        "value": pairs / sec / 1e9, "unit": "Gpair-evals/s", "ms_per_step": 1e3 * sec,
+        "config": {"workload": "te_32series_100000frames_h0.2_tau1-2-5-10",
+                   "jobs": f"{jobs_3d} 3-D + {jobs_2d} 2-D + {jobs_1d} 1-D densities, one persistent launch",
+                   "tuning_q_m0_mr_group": list(tun)},
+        "pairs_executed": pairs, "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
+        "dense_equiv_gpair_per_s": dense / sec / 1e9,
+        "roofline": roofline(st["pairs_by_d"], kern_ms),
+    }
+    # the reference runs one lag per invocation: per lag 32 1-D + 32 lagged
+    # 2-D + 496 2-D + 992 3-D combined_densities calls (transs.cpp:42-74)
+    calls = {1: n_cols * len(taus), 2: (n_cols + n_pairs) * len(taus), 3: 2 * n_pairs * len(taus)}
+    if not args.no_plain:
+        log("timing plain per-pair path")
+        tb = {}
+        for d, cols, tau in ((1, [0], None), (2, [0, 1], None), (3, [0, 0, 1], [0, 5, 0])):
+            hh = [h] * d
+            combined_densities(X, n_rows, cols, hh, tau, method="brute", ctx=ctx)
+            tb[d] = time_call(lambda: combined_densities(X, n_rows, cols, hh, tau, method="brute", ctx=ctx))
+        total = sum(calls[d] * tb[d] for d in (1, 2, 3))
+        res["plain_path"] = {"brute": {f"t_{d}d_ms": 1e3 * tb[d] for d in (1, 2, 3)} | {"te_ms": 1e3 * total}}
+        res["speedup_vs_plain_brute"] = total / sec
+    dx.free()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        th = args.cpu_threads or host_cores()
+        tc = {1: cpu_omp_time(X[0:1], n_rows, [h], th), 2: cpu_omp_time(X[0:2], n_rows, [h, h], th)}
+        lag = np.stack([X[0, :n_rows - 5], X[0, 5:], X[1, :n_rows - 5]])
+        tc[3] = cpu_omp_time(lag, n_rows - 5, [h, h, h], th)
+        total = sum(calls[d] * tc[d] for d in (1, 2, 3))
+        res["cpu_baseline"] = {
+            "value": total, "unit": "s per 4-lag TE matrix (extrapolated)", "cores": th, "kind": "port",
+            "sample": "one 1-D, 2-D and 3-D density at N=100000 ({:.2f}, {:.2f}, {:.2f} s), scaled by the "
+                      "reference's per-lag call counts".format(tc[1], tc[2], tc[3]),
+        }
+        res["speedup_vs_cpu"] = total / sec
+    return res
+
+
+# ---------------------------------------------------------------- uneven
+def run_uneven(args) -> dict:
+    from entropical_amd import Context, batch_autotune, batch_entropy, datasets
+
+    log("generating 500 uneven trajectories")
+    buf, jobs = datasets.uneven_batch(500)
+    ctx = Context(0)
+    ptr = ctx.alloc(buf.nbytes)
+    ctx.h2d(ptr, buf)
+    tun = batch_autotune(ctx, ptr, jobs) if not args.no_tune else ctx.get_tuning()
+    log(f"tuning (q, m0, mr, group_h) = {tun}")
+    sec, kern_ms, st, util = timed_steps(ctx, lambda: batch_entropy(ctx, ptr, jobs), args.steps, args.warmup)
+    pairs = sum(st["pairs_by_d"])
+    dense = float(sum(j["n"] ** 2 for j in jobs))
+    log(f"uneven batch {1e3 * sec:.1f} ms/step, density kernel {kern_ms:.1f} ms, {pairs:.3e} pairs")
+    ctx.free(ptr)
+    ctx.close()
+    return {
+        "metric": "Gpair-evals/s (500 uneven trajectories, densities + entropies)",
+        "value": pairs / sec / 1e9, "unit": "Gpair-evals/s", "ms_per_step": 1e3 * sec,
+        "config": {"workload": "uneven_500traj_2k-500k_d1-3", "points": int(sum(j["n"] for j in jobs)),
+                   "tuning_q_m0_mr_group": list(tun)},
+        "pairs_executed": pairs, "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
+        "dense_equiv_gpair_per_s": dense / sec / 1e9,
+        "roofline": roofline(st["pairs_by_d"], kern_ms),
+    }
+
+
+# ------------------------------------------------------------- entropy1d
+def run_entropy1d(args) -> dict:
+    from entropical_amd import Context, batch_autotune, batch_entropy, combined_densities, datasets
+
+    n, h = 100_000, 0.1
+    x = datasets.gmm_1d(n)
+    ctx = Context(0)
+    ptr = ctx.alloc(x.nbytes)
+    ctx.h2d(ptr, x)
+    spec = [{"n_dim": 1, "n": n, "n_rows": n, "off": [0], "h": [h]}]
+    tun = batch_autotune(ctx, ptr, spec) if not args.no_tune else ctx.get_tuning()
+    sec, kern_ms, st, util = timed_steps(ctx, lambda: batch_entropy(ctx, ptr, spec), args.steps, args.warmup)
+    H = float(batch_entropy(ctx, ptr, spec)[0])
+    pairs = sum(st["pairs_by_d"])
+    res = {
+        "metric": "Gpair-evals/s (1-D KDE entropy, N=100k)", "value": pairs / sec / 1e9, "unit": "Gpair-evals/s",
+        "ms_per_step": 1e3 * sec,
+        "config": {"workload": "entropy1d_gmm3_100000_h0.1", "tuning_q_m0_mr_group": list(tun)},
+        "entropy": H, "pairs_executed": pairs, "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
+        "dense_equiv_gpair_per_s": float(n) ** 2 / sec / 1e9, "roofline": roofline(st["pairs_by_d"], kern_ms),
+    }
+    if not args.no_plain:
+        combined_densities(x, n, [0], [h], method="brute", ctx=ctx)
+        tb = time_call(lambda: combined_densities(x, n, [0], [h], method="brute", ctx=ctx))
+        res["plain_path"] = {"brute": {"t_1d_ms": 1e3 * tb}}
+        res["speedup_vs_plain_brute"] = tb / sec
+    ctx.free(ptr)
+    ctx.close()
+    if not args.no_cpu_baseline:
+        th = args.cpu_threads or host_cores()
+        t = cpu_omp_time(x, n, [h], th)
+        res["cpu_baseline"] = {"value": t, "unit": "s per density", "cores": th, "kind": "port",
+                               "sample": "the full workload (one 1-D density, N=100000)"}
+        res["speedup_vs_cpu"] = t / sec
+    return res
+
+
+WORKLOADS = {"mi": run_mi, "te": run_te, "uneven": run_uneven, "entropy1d": run_entropy1d}
 
 
 def main(argv=None) -> int:
@@ -163,8 +292,10 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="mi")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -176,7 +307,7 @@ def main(argv=None) -> int:
             file=sys.stderr,
         )
         return 2
-    res = run_mi(args)
+    res = WORKLOADS[args.workload](args)
     line = {
         "metric": res.pop("metric"),
         "value": res.pop("value"),
@@ -188,7 +319,7 @@ def main(argv=None) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": res.pop("dtype"),
+        "dtype": "f32",
         "data": "synthetic (seeded generator)",
         "config": res.pop("config"),
     }
