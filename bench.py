@@ -58,7 +58,30 @@ def time_call(fn, reps: int = 3) -> float:
     return float(np.median(ts))
 
 
-def roofline(pairs_by_d, kern_ms: float) -> dict:
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per k_density launch for this workload from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by the factor the same counter showed on a copy of known size),
+    or None when no summary exists for the workload."""
+    path = os.path.join(PROFILES, f"k_density_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    with open(path) as f:
+        for line in f:
+            if "=" in line and not line.lstrip().startswith("#"):
+                k, v = line.split("=", 1)
+                kv[k.strip()] = v.strip()
+    try:
+        return int(float(kv["traffic_bytes"]))
+    except (KeyError, ValueError):
+        return None
+
+
+def roofline(pairs_by_d, kern_ms: float, workload: str) -> dict:
     flops = sum(p * f for p, f in zip(pairs_by_d, FLOPS_PER_PAIR))
     achieved = flops / (kern_ms * 1e-3) / 1e12
     return {
@@ -68,10 +91,29 @@ def roofline(pairs_by_d, kern_ms: float) -> dict:
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_TFLOPS,
-        "traffic": None,
+        "traffic": measured_traffic(workload),
         "kernel_ms": kern_ms,
         "flops_per_pair": "4d (d=1: 4, d=2: 8, d=3: 12)",
     }
+
+
+def copy_calibration() -> dict:
+    """A plain streaming kernel of known traffic (1 GiB read + 1 GiB written
+    per launch, two launches) to scale the FETCH_SIZE / WRITE_SIZE counters
+    of a rocprofv3 --pmc run of this same command."""
+    import torch
+
+    n = 256 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 2
+    return {"calibration": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 2,
+            "ms_per_launch": 1e3 * dt}
 
 
 def timed_steps(ctx, step, steps: int, warmup: int):
@@ -130,7 +172,7 @@ def run_mi(args) -> dict:
         "pairs_by_dim": st["pairs_by_d"],
         "lane_utilisation": util,
         "dense_equiv_gpair_per_s": 2080 * n_rows * n_rows / sec / 1e9,
-        "roofline": roofline(st["pairs_by_d"], kern_ms),
+        "roofline": roofline(st["pairs_by_d"], kern_ms, "mi"),
     }
     if not args.no_plain:
         log("timing plain per-pair path (one call each, extrapolated to mi.cpp's 4096 calls)")
@@ -187,7 +229,7 @@ def run_te(args) -> dict:
                    "tuning_q_m0_mr_group": list(tun)},
         "pairs_executed": pairs, "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
         "dense_equiv_gpair_per_s": dense / sec / 1e9,
-        "roofline": roofline(st["pairs_by_d"], kern_ms),
+        "roofline": roofline(st["pairs_by_d"], kern_ms, "te"),
     }
     # the reference runs one lag per invocation: per lag 32 1-D + 32 lagged
     # 2-D + 496 2-D + 992 3-D combined_densities calls (transs.cpp:42-74)
@@ -243,7 +285,7 @@ def run_uneven(args) -> dict:
                    "tuning_q_m0_mr_group": list(tun)},
         "pairs_executed": pairs, "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
         "dense_equiv_gpair_per_s": dense / sec / 1e9,
-        "roofline": roofline(st["pairs_by_d"], kern_ms),
+        "roofline": roofline(st["pairs_by_d"], kern_ms, "uneven"),
     }
 
 
@@ -266,7 +308,7 @@ def run_entropy1d(args) -> dict:
         "ms_per_step": 1e3 * sec,
         "config": {"workload": "entropy1d_gmm3_100000_h0.1", "tuning_q_m0_mr_group": list(tun)},
         "entropy": H, "pairs_executed": pairs, "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
-        "dense_equiv_gpair_per_s": float(n) ** 2 / sec / 1e9, "roofline": roofline(st["pairs_by_d"], kern_ms),
+        "dense_equiv_gpair_per_s": float(n) ** 2 / sec / 1e9, "roofline": roofline(st["pairs_by_d"], kern_ms, "entropy1d"),
     }
     if not args.no_plain:
         combined_densities(x, n, [0], [h], method="brute", ctx=ctx)
@@ -297,7 +339,12 @@ def main(argv=None) -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a copy kernel of known traffic (counter calibration) and exit")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        print(json.dumps(copy_calibration()), flush=True)
+        return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(

@@ -185,6 +185,17 @@ int ent_ctx_set_graph(ent_ctx* ctx, int enable) {
   return ENT_OK;
 }
 
+int ent_ctx_last_sort(ent_ctx* ctx, uint32_t* out2) {
+  CAPI_TRY(check_ctx(ctx));
+  if (!out2) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  out2[0] = ctx->eng->last.radix_passes;
+  out2[1] = ctx->eng->last.sort_bases;
+  return ENT_OK;
+}
+
 int ent_ctx_last_pairs(ent_ctx* ctx, uint64_t* out3) {  // out3: 4 entries
   CAPI_TRY(check_ctx(ctx));
   if (!out3) {

@@ -4,8 +4,11 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include "kde_kernels.hpp"
 
 namespace ent {
@@ -52,7 +55,8 @@ Engine::~Engine() {
   if (stream_) hipStreamSynchronize(stream_);
   DevBuf* bufs[] = {&d_jobs,  &d_chunks, &d_chunk_first, &d_mm,     &d_flags,        &d_keys[0],
                     &d_keys[1], &d_idx[0], &d_idx[1],     &d_hist,   &d_scan_scratch, &d_scoord,
-                    &d_tile_cnt, &d_tiles, &d_counter, &d_cellstart};
+                    &d_tile_cnt, &d_tiles, &d_counter, &d_cellstart, &d_bjobs, &d_bchunks,
+                    &d_bchunk_first, &d_bbase, &d_bkeys[0], &d_bkeys[1], &d_bidx[0], &d_bidx[1]};
   for (GraphCache* c : {&pre_graphs_, &dens_graphs_})
     for (auto& e : c->entries) hipGraphExecDestroy(e.second);
   for (DevBuf* b : bufs)
@@ -235,58 +239,173 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     }
     max_key = std::max<uint64_t>(max_key, cells - 1);
   }
+  auto passes_for = [](uint64_t max_value) {
+    int bits = 0;
+    while (bits < 64 && (max_value >> bits) != 0) ++bits;
+    return (bits + kRadixBits - 1) / kRadixBits;
+  };
   int bits = 0;
   while (bits < 64 && (max_key >> bits) != 0) ++bits;
-  const int npass = (bits + kRadixBits - 1) / kRadixBits;
-  last.radix_passes = (uint32_t)npass;
+  const int npass_full = passes_for(max_key);
   const bool wide = bits > 32;
   keys_wide = wide;
-  std::memcpy(hs_jobs, h_jobs.data(), sizeof(DJob) * J);
-  ENT_HIP(hipMemcpyAsync(d_jobs.p, hs_jobs, sizeof(DJob) * J, hipMemcpyHostToDevice, stream_));
+
+  // Two-stage sort: jobs with the same dim-0 window (offset, length) and
+  // dim-0 bandwidth have the same dim-0 boxes; one stable sort of that window
+  // by dim-0 box (a base) serves all of them, and each job then needs only
+  // the passes over its remaining dims' key (k_major_keys).  Chosen when the
+  // bases hold at most half the batch's points; ENT_SORT=full|shared forces
+  // a mode (both give the same permutation).
+  h_bjobs.clear();
+  h_bchunks.clear();
+  h_bchunk_first.clear();
+  h_bbase.assign(J, 0);
+  uint64_t PB = 0, max_b0 = 0, max_major = 0;
+  {
+    std::map<std::tuple<uint64_t, uint32_t, uint32_t>, uint32_t> seen;
+    for (uint32_t j = 0; j < J; ++j) {
+      const DJob& D = h_jobs[j];
+      uint32_t hbits;
+      std::memcpy(&hbits, &D.h[0], 4);
+      auto ins = seen.emplace(std::make_tuple(D.off[0], D.n, hbits), (uint32_t)h_bjobs.size());
+      if (ins.second) {
+        DJob B = D;
+        B.d = 1;
+        for (int e = 1; e < 3; ++e) {
+          B.off[e] = 0;
+          B.h[e] = B.hinv[e] = B.bs[e] = 1.0f;
+          B.mn[e] = 0.0f;
+          B.nb[e] = 1;
+        }
+        B.flags = 0;
+        B.cells = D.nb[0];
+        B.cbase = kNoTable;
+        B.pbase = PB;
+        const uint32_t bi = (uint32_t)h_bjobs.size();
+        h_bchunk_first.push_back((uint32_t)h_bchunks.size());
+        for (uint32_t st = 0; st < D.n; st += kChunk) h_bchunks.push_back(make_uint2(bi, st));
+        PB += D.n;
+        h_bjobs.push_back(B);
+      }
+      h_bbase[j] = h_bjobs[ins.first->second].pbase;
+      max_b0 = std::max<uint64_t>(max_b0, D.nb[0] - 1);
+      max_major = std::max<uint64_t>(max_major, D.cells / D.nb[0] - 1);
+    }
+    h_bchunk_first.push_back((uint32_t)h_bchunks.size());
+  }
+  bool shared = 2 * PB <= P;
+  if (const char* m = std::getenv("ENT_SORT")) {
+    if (!std::strcmp(m, "full")) shared = false;
+    if (!std::strcmp(m, "shared")) shared = true;
+  }
+  bool staged = true;
+  if (const char* m = std::getenv("ENT_SCATTER"))
+    if (!std::strcmp(m, "direct")) staged = false;
+  const uint32_t NB = (uint32_t)h_bjobs.size();
+  const uint32_t NCB = (uint32_t)h_bchunks.size();
+  const int npass0 = shared ? passes_for(max_b0) : 0;
+  const int npass = shared ? passes_for(max_major) : npass_full;
+  last.radix_passes = (uint32_t)(npass0 + npass);
+  last.sort_bases = shared ? NB : 0;
+
+  // upload the final job table (and the base tables)
+  const size_t base_bytes = shared ? sizeof(DJob) * NB + sizeof(uint2) * NCB + sizeof(uint32_t) * (NB + 1) +
+                                         sizeof(uint64_t) * J
+                                   : 0;
+  ENT_TRY(ensure_host(sizeof(DJob) * J + base_bytes + 64));
+  {
+    char* p = (char*)host_stage_;
+    std::memcpy(p, h_jobs.data(), sizeof(DJob) * J);
+    ENT_HIP(hipMemcpyAsync(d_jobs.p, p, sizeof(DJob) * J, hipMemcpyHostToDevice, stream_));
+    p += sizeof(DJob) * J;
+    if (shared) {
+      ENT_TRY(ensure(d_bjobs, sizeof(DJob) * NB));
+      ENT_TRY(ensure(d_bchunks, sizeof(uint2) * NCB));
+      ENT_TRY(ensure(d_bchunk_first, sizeof(uint32_t) * (NB + 1)));
+      ENT_TRY(ensure(d_bbase, sizeof(uint64_t) * J));
+      auto up = [&](DevBuf& b, const void* src, size_t n) {
+        std::memcpy(p, src, n);
+        hipError_t e = hipMemcpyAsync(b.p, p, n, hipMemcpyHostToDevice, stream_);
+        p += n;
+        return e;
+      };
+      ENT_HIP(up(d_bjobs, h_bjobs.data(), sizeof(DJob) * NB));
+      ENT_HIP(up(d_bchunks, h_bchunks.data(), sizeof(uint2) * NCB));
+      ENT_HIP(up(d_bchunk_first, h_bchunk_first.data(), sizeof(uint32_t) * (NB + 1)));
+      ENT_HIP(up(d_bbase, h_bbase.data(), sizeof(uint64_t) * J));
+    }
+  }
   if (stop == kStopAfterPlan) return kOk;
 
   // buffers sized by the plan
   const uint64_t HM = (uint64_t)kRadixBins * NC;
-  const uint64_t scan_m = std::max<uint64_t>(HM, NC);
+  const uint64_t HMB = (uint64_t)kRadixBins * NCB;
+  const uint64_t scan_m = std::max<uint64_t>(std::max<uint64_t>(HM, NC), shared ? HMB : 0);
   const size_t key_bytes = wide ? sizeof(uint64_t) : sizeof(uint32_t);
   ENT_TRY(ensure(d_keys[0], key_bytes * P));
   ENT_TRY(ensure(d_keys[1], key_bytes * P));
   ENT_TRY(ensure(d_idx[0], sizeof(uint32_t) * P));
   ENT_TRY(ensure(d_idx[1], sizeof(uint32_t) * P));
-  ENT_TRY(ensure(d_hist, sizeof(uint32_t) * (HM + 1)));
+  ENT_TRY(ensure(d_hist, sizeof(uint32_t) * (scan_m + 1)));
   ENT_TRY(ensure(d_scan_scratch, sizeof(uint32_t) * scan_scratch_elems(scan_m)));
   ENT_TRY(ensure(d_scoord, sizeof(float) * 3 * P));
   ENT_TRY(ensure(d_tile_cnt, sizeof(uint32_t) * (NC + 1)));
   ENT_TRY(ensure(d_tiles, sizeof(DTile) * P));
   ENT_TRY(ensure(d_counter, 64));
   ENT_TRY(ensure(d_cellstart, sizeof(uint32_t) * std::max<uint64_t>(table_total, 1)));
+  if (shared) {
+    for (int b = 0; b < 2; ++b) {
+      ENT_TRY(ensure(d_bkeys[b], key_bytes * PB));
+      ENT_TRY(ensure(d_bidx[b], sizeof(uint32_t) * PB));
+    }
+  }
   last.table_cells = table_total;
 
   void* keys[2] = {d_keys[0].p, d_keys[1].p};
   uint32_t* idx[2] = {(uint32_t*)d_idx[0].p, (uint32_t*)d_idx[1].p};
   uint32_t* hist = (uint32_t*)d_hist.p;
   uint32_t* scratch = (uint32_t*)d_scan_scratch.p;
+  float* scoord = (float*)d_scoord.p;
 
   sorted_buf = npass & 1;
   const uint32_t tq = (uint32_t)(kWave * tuning.q);
   uint32_t* tcnt = (uint32_t*)d_tile_cnt.p;
+  // stable LSD passes over `np` digits of a chunked batch; the last one of
+  // the main batch gathers the coordinates and writes full keys
+  auto radix = [&](const DJob* jb, const uint2* cb, const uint32_t* cf, uint32_t nc, void** kk, uint32_t** ii,
+                   int np, float* sc_last) {
+    const uint64_t hm = (uint64_t)kRadixBins * nc;
+    int cur = 0;
+    for (int pass = 0; pass < np; ++pass) {
+      const int shift = pass * kRadixBits;
+      launch_radix_hist(stream_, wide, jb, cb, cf, nc, kk[cur], shift, hist);
+      launch_exclusive_scan(stream_, hist, hist, hm, scratch);
+      launch_radix_scatter(stream_, wide, staged, jb, cb, cf, nc, kk[cur], ii[cur], kk[cur ^ 1], ii[cur ^ 1],
+                           shift, hist, X, pass == np - 1 ? sc_last : nullptr);
+      cur ^= 1;
+    }
+    return cur;
+  };
   // keys -> radix passes [-> gather -> tiles]; a fixed launch sequence for a
   // given plan shape, replayed from a captured graph when the shape repeats
   auto issue_pre = [&](bool with_tiles) {
-    launch_cell_keys(stream_, wide, dj, dch, NC, X, keys[0], idx[0]);
     int cur = 0;
-    for (int pass = 0; pass < npass; ++pass) {
-      int shift = pass * kRadixBits;
-      // the last pass also writes the sorted coordinates
-      float* sc = (with_tiles && pass == npass - 1) ? (float*)d_scoord.p : nullptr;
-      launch_radix_hist(stream_, wide, dj, dch, dcf, NC, keys[cur], shift, hist);
-      launch_exclusive_scan(stream_, hist, hist, HM, scratch);
-      launch_radix_scatter(stream_, wide, dj, dch, dcf, NC, keys[cur], idx[cur], keys[cur ^ 1], idx[cur ^ 1],
-                           shift, hist, X, sc);
-      cur ^= 1;
+    if (shared) {
+      const DJob* dbj = (const DJob*)d_bjobs.p;
+      void* bk[2] = {d_bkeys[0].p, d_bkeys[1].p};
+      uint32_t* bi[2] = {(uint32_t*)d_bidx[0].p, (uint32_t*)d_bidx[1].p};
+      launch_cell_keys(stream_, wide, dbj, (const uint2*)d_bchunks.p, NCB, X, bk[0], bi[0]);
+      const int bc = radix(dbj, (const uint2*)d_bchunks.p, (const uint32_t*)d_bchunk_first.p, NCB, bk, bi,
+                           npass0, nullptr);
+      launch_major_keys(stream_, wide, dj, dch, NC, (const uint64_t*)d_bbase.p, bi[bc], X, keys[0], idx[0],
+                        npass == 0 ? scoord : nullptr);
+      cur = radix(dj, dch, dcf, NC, keys, idx, npass, scoord);
+    } else {
+      launch_cell_keys(stream_, wide, dj, dch, NC, X, keys[0], idx[0]);
+      cur = radix(dj, dch, dcf, NC, keys, idx, npass, with_tiles ? scoord : nullptr);
+      if (with_tiles && npass == 0) launch_gather_sorted(stream_, dj, dch, NC, X, idx[cur], scoord);
     }
     if (!with_tiles) return;
-    if (npass == 0) launch_gather_sorted(stream_, dj, dch, NC, X, idx[cur], (float*)d_scoord.p);
     if (table_total) launch_cell_starts(stream_, wide, dj, dch, NC, keys[cur], (uint32_t*)d_cellstart.p);
     launch_tile_count(stream_, wide, dj, dch, NC, keys[cur], tq, tcnt);
     launch_exclusive_scan(stream_, tcnt, tcnt, NC, scratch);
@@ -299,6 +418,9 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     return kOk;
   }
   PlanKey key = plan_key(X, dens, J, NC, P, npass, false);
+  key.NCB = shared ? NCB : 0;
+  key.PB = shared ? PB : 0;
+  key.variant = (shared ? 1 : 0) | (staged ? 2 : 0) | (npass0 << 2);
   ENT_TRY(run_captured(pre_graphs_, key, [&] { issue_pre(true); }));
   return launch_density_common(false, X, dens, P);
 }

@@ -48,13 +48,13 @@ struct DevBuf {
 struct PlanKey {
   const void* X = nullptr;
   const void* dens = nullptr;
-  uint64_t P = 0, gen = 0;
-  uint32_t J = 0, NC = 0;
-  int npass = 0, raw = 0, q = 0, m0 = 0, mr = 0, gh = 0, grid = 0;
+  uint64_t P = 0, gen = 0, PB = 0;
+  uint32_t J = 0, NC = 0, NCB = 0;
+  int npass = 0, raw = 0, q = 0, m0 = 0, mr = 0, gh = 0, grid = 0, variant = 0;
   bool operator==(const PlanKey& o) const {
-    return X == o.X && dens == o.dens && P == o.P && gen == o.gen && J == o.J && NC == o.NC &&
-           npass == o.npass && raw == o.raw && q == o.q && m0 == o.m0 && mr == o.mr && gh == o.gh &&
-           grid == o.grid;
+    return X == o.X && dens == o.dens && P == o.P && gen == o.gen && PB == o.PB && J == o.J &&
+           NC == o.NC && NCB == o.NCB && npass == o.npass && raw == o.raw && q == o.q && m0 == o.m0 &&
+           mr == o.mr && gh == o.gh && grid == o.grid && variant == o.variant;
   }
 };
 
@@ -67,7 +67,8 @@ struct RunStats {
   uint64_t pairs_by_d[3] = {0, 0, 0};  // the same, per job dimension
   uint64_t lane_pairs = 0;   // lane-slot evaluations issued (incl. idle lanes / padding)
   uint64_t tiles = 0;
-  uint32_t radix_passes = 0;
+  uint32_t radix_passes = 0;  // base passes + job passes
+  uint32_t sort_bases = 0;    // shared dim-0 sorts (0: one sort per job by full key)
   uint64_t table_cells = 0;  // cell-start table entries of the last plan
 };
 
@@ -136,10 +137,16 @@ class Engine {
   // workspace buffers
   DevBuf d_jobs, d_chunks, d_chunk_first, d_mm, d_flags, d_keys[2], d_idx[2], d_hist,
       d_scan_scratch, d_scoord, d_tile_cnt, d_tiles, d_counter, d_cellstart;
+  // two-stage sort: base (shared dim-0 sort) tables and buffers
+  DevBuf d_bjobs, d_bchunks, d_bchunk_first, d_bbase, d_bkeys[2], d_bidx[2];
   // host copies
   std::vector<DJob> h_jobs;
   std::vector<uint2> h_chunks;
   std::vector<uint32_t> h_chunk_first;
+  std::vector<DJob> h_bjobs;
+  std::vector<uint2> h_bchunks;
+  std::vector<uint32_t> h_bchunk_first;
+  std::vector<uint64_t> h_bbase;  // per job: first slot of its base
   uint64_t total_points = 0;
   int sorted_buf = 0;  // which of d_keys/d_idx holds the sorted result
   bool keys_wide = false;  // cell keys stored as uint64 (else uint32)

@@ -57,6 +57,22 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
+// Workgroups are dispatched round-robin over the XCDs (workgroup b runs on
+// XCD b % 8).  This bijection hands each XCD a contiguous share of the
+// workgroup-indexed work list, so the few jobs one XCD works on at a time
+// keep their coordinate columns in that XCD's own L2 instead of every XCD
+// touching every active job.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t G) {
+#ifdef ENT_NO_XCD_ORDER
+  (void)G;
+  return b;
+#else
+  const uint32_t q = G / kXcds, r = G % kXcds, x = b % kXcds;
+  return x * q + (x < r ? x : r) + b / kXcds;
+#endif
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const int l = lane_id();
   return l == 0 ? 0ull : (~0ull >> (64 - l));
@@ -249,11 +265,168 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     wave_sync();
     if (valid) {
       uint32_t dst = base + rank;
-      keys_out[dst] = key;
       idx_out[dst] = id;
       if constexpr (GATHER) {
         const uint64_t p = dst - J.pbase;
-        for (uint32_t e = 0; e < J.d; ++e) scoord[3 * J.pbase + (uint64_t)e * J.n + p] = X[J.off[e] + id];
+        float v[3] = {0.f, 0.f, 0.f};
+        for (uint32_t e = 0; e < J.d; ++e) {
+          v[e] = X[J.off[e] + id];
+          scoord[3 * J.pbase + (uint64_t)e * J.n + p] = v[e];
+        }
+        keys_out[dst] = (KeyT)cell_key(J, v);
+      } else {
+        keys_out[dst] = key;
+      }
+    }
+  }
+}
+
+// LDS-staged stable scatter, one wavefront per block and chunk.  Phase 1
+// ranks the chunk's elements by digit (stable, as above) into LDS, so the
+// chunk sits there sorted by digit; phase 2 streams it out in that order:
+// consecutive lanes then write consecutive global slots within each digit's
+// run, instead of 64 unrelated addresses per store.  The chunk's per-digit
+// counts come from the scanned offsets: offs[i + 1] - offs[i] (the scan is
+// contiguous over all jobs and ends with the grand total at offs[m]).
+// With GATHER (last pass) the full cell key is recomputed from the gathered
+// coordinates, so a pass over a partial key (the two-stage sort) also ends
+// with full keys.
+template <typename KeyT, bool GATHER>
+__global__ __launch_bounds__(kWave) void k_radix_scatter_lds(
+    const DJob* __restrict__ jobs, const uint2* __restrict__ chunks,
+    const uint32_t* __restrict__ chunk_first, uint32_t n_chunks,
+    const KeyT* __restrict__ keys_in, const uint32_t* __restrict__ idx_in,
+    KeyT* __restrict__ keys_out, uint32_t* __restrict__ idx_out, int shift,
+    const uint32_t* __restrict__ offs, const float* __restrict__ X, float* __restrict__ scoord) {
+  __shared__ uint32_t lrun[kRadixBins];
+  __shared__ uint32_t delta[kRadixBins];
+  __shared__ KeyT skey[kChunk];
+  __shared__ uint32_t sid[kChunk];
+  const int l = lane_id();
+  const uint32_t c = GATHER ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x;
+  if (c >= n_chunks) return;
+  const uint2 ch = chunks[c];
+  const DJob& J = jobs[ch.x];
+  const uint32_t c0 = chunk_first[ch.x];
+  const uint32_t nch = chunk_first[ch.x + 1] - c0;
+  // digits 4l .. 4l+3: global start, chunk count, chunk-local start
+  {
+    constexpr int kPer = kRadixBins / kWave;
+    uint32_t g[kPer], cnt[kPer], s = 0;
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) {
+      const uint64_t i = (uint64_t)kRadixBins * c0 + (uint64_t)(kPer * l + t) * nch + (c - c0);
+      g[t] = offs[i];
+      cnt[t] = offs[i + 1] - g[t];
+      s += cnt[t];
+    }
+    uint32_t ls = wave_incl_scan(s) - s;
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) {
+      lrun[kPer * l + t] = ls;
+      delta[kPer * l + t] = g[t] - ls;
+      ls += cnt[t];
+    }
+  }
+  const uint32_t nv = min((uint32_t)kChunk, J.n - ch.y);
+  KeyT kr[kChunkItems];
+  uint32_t ir[kChunkItems];
+#pragma unroll
+  for (int i = 0; i < kChunkItems; ++i) {
+    const uint32_t k = ch.y + i * kWave + l;
+    kr[i] = 0;
+    ir[i] = 0;
+    if (k < J.n) {
+      kr[i] = keys_in[J.pbase + k];
+      ir[i] = idx_in[J.pbase + k];
+    }
+  }
+  wave_sync();
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int i = 0; i < kChunkItems; ++i) {
+    const bool valid = (uint32_t)(i * kWave + l) < nv;
+    const uint32_t dg = (uint32_t)(kr[i] >> shift) & (kRadixBins - 1);
+    uint64_t match = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kRadixBits; ++b) {
+      const bool bit = (dg >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      match &= bit ? bal : ~bal;
+    }
+    const uint32_t rank = __popcll(match & lt);
+    const uint32_t base = valid ? lrun[dg] : 0u;
+    wave_sync();
+    if (valid && rank == 0) lrun[dg] = base + (uint32_t)__popcll(match);
+    if (valid) {
+      skey[base + rank] = kr[i];
+      sid[base + rank] = ir[i];
+    }
+    wave_sync();
+  }
+  for (uint32_t j = l; j < nv; j += kWave) {
+    const KeyT key = skey[j];
+    const uint32_t id = sid[j];
+    const uint32_t dst = delta[(uint32_t)(key >> shift) & (kRadixBins - 1)] + j;
+    idx_out[dst] = id;
+    if constexpr (GATHER) {
+      const uint64_t p = dst - J.pbase;
+      float v[3] = {0.f, 0.f, 0.f};
+      for (uint32_t e = 0; e < J.d; ++e) {
+        v[e] = X[J.off[e] + id];
+        scoord[3 * J.pbase + (uint64_t)e * J.n + p] = v[e];
+      }
+      keys_out[dst] = (KeyT)cell_key(J, v);
+    } else {
+      keys_out[dst] = key;
+    }
+  }
+}
+
+// ------------------------------------------------ two-stage sort, stage 2
+// Jobs that share their dim-0 window and bandwidth share one sort of that
+// window by dim-0 box (the base, stable from the identity order).  Each job
+// starts from its base's order and is then sorted stably by the key of its
+// remaining dims, r = b1 + b2*nb1; stability keeps the dim-0 order inside
+// equal r, so the result is the order by full key b0 + r*nb0 with ties in
+// point order -- the same permutation as one sort by the full key.
+// Writes (r, point) at the job's own positions.  With scoord (no pass over r
+// follows: every r is 0) it writes the full key and the sorted coordinates.
+template <typename KeyT>
+__global__ __launch_bounds__(kBlock) void k_major_keys(const DJob* __restrict__ jobs,
+                                                       const uint2* __restrict__ chunks,
+                                                       uint32_t n_chunks,
+                                                       const uint64_t* __restrict__ bbase,
+                                                       const uint32_t* __restrict__ bidx,
+                                                       const float* __restrict__ X,
+                                                       KeyT* __restrict__ keys,
+                                                       uint32_t* __restrict__ idx,
+                                                       float* __restrict__ scoord) {
+  const uint32_t c = xcd_order(blockIdx.x, gridDim.x) * (kBlock / kWave) + threadIdx.x / kWave;
+  if (c >= n_chunks) return;
+  const uint2 ch = chunks[c];
+  const DJob J = jobs[ch.x];
+  const uint32_t* bi = bidx + bbase[ch.x];
+  const int l = lane_id();
+  for (int i = 0; i < kChunkItems; ++i) {
+    const uint32_t k = ch.y + i * kWave + l;
+    if (k < J.n) {
+      const uint32_t id = bi[k];
+      idx[J.pbase + k] = id;
+      if (scoord) {
+        float v[3] = {0.f, 0.f, 0.f};
+        for (uint32_t e = 0; e < J.d; ++e) {
+          v[e] = X[J.off[e] + id];
+          scoord[3 * J.pbase + (uint64_t)e * J.n + k] = v[e];
+        }
+        keys[J.pbase + k] = (KeyT)cell_key(J, v);
+      } else {
+        uint64_t r = 0;
+        if (J.d >= 2) {
+          r = box_of(X[J.off[1] + id], J.mn[1], J.bs[1], J.nb[1]);
+          if (J.d == 3) r += (uint64_t)box_of(X[J.off[2] + id], J.mn[2], J.bs[2], J.nb[2]) * J.nb[1];
+        }
+        keys[J.pbase + k] = (KeyT)r;
       }
     }
   }
@@ -829,14 +1002,20 @@ void launch_radix_hist(hipStream_t s, bool wide, const DJob* jobs, const uint2* 
                        chunk_first, n_chunks, (const uint32_t*)keys, shift, hist);
 }
 
-void launch_radix_scatter(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks,
+void launch_radix_scatter(hipStream_t s, bool wide, bool staged, const DJob* jobs, const uint2* chunks,
                           const uint32_t* chunk_first, uint32_t n_chunks, const void* kin,
                           const uint32_t* iin, void* kout, uint32_t* iout, int shift, const uint32_t* offs,
                           const float* X, float* scoord) {
   const bool gather = scoord != nullptr;
-#define ENT_SCATTER(KT, G)                                                                                   \
-  hipLaunchKernelGGL((k_radix_scatter<KT, G>), chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,       \
-                     chunk_first, n_chunks, (const KT*)kin, iin, (KT*)kout, iout, shift, offs, X, scoord)
+#define ENT_SCATTER(KT, G)                                                                                     \
+  do {                                                                                                         \
+    if (staged)                                                                                                \
+      hipLaunchKernelGGL((k_radix_scatter_lds<KT, G>), dim3(n_chunks), dim3(kWave), 0, s, jobs, chunks,      \
+                         chunk_first, n_chunks, (const KT*)kin, iin, (KT*)kout, iout, shift, offs, X, scoord); \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_radix_scatter<KT, G>), chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,   \
+                         chunk_first, n_chunks, (const KT*)kin, iin, (KT*)kout, iout, shift, offs, X, scoord); \
+  } while (0)
   if (wide) {
     if (gather) ENT_SCATTER(uint64_t, true);
     else ENT_SCATTER(uint64_t, false);
@@ -845,6 +1024,17 @@ void launch_radix_scatter(hipStream_t s, bool wide, const DJob* jobs, const uint
     else ENT_SCATTER(uint32_t, false);
   }
 #undef ENT_SCATTER
+}
+
+void launch_major_keys(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
+                       const uint64_t* bbase, const uint32_t* bidx, const float* X, void* keys,
+                       uint32_t* idx, float* scoord) {
+  if (wide)
+    hipLaunchKernelGGL(k_major_keys<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
+                       n_chunks, bbase, bidx, X, (uint64_t*)keys, idx, scoord);
+  else
+    hipLaunchKernelGGL(k_major_keys<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
+                       n_chunks, bbase, bidx, X, (uint32_t*)keys, idx, scoord);
 }
 
 uint64_t scan_scratch_elems(uint64_t m) {

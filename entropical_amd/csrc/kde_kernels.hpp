@@ -17,11 +17,18 @@ void launch_bin_ids(hipStream_t s, const DJob* jobs, const float* X, int32_t* bo
 void launch_radix_hist(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks,
                        const uint32_t* chunk_first, uint32_t n_chunks, const void* keys, int shift,
                        uint32_t* hist);
-// scoord != nullptr: also write sorted coordinates (last pass)
-void launch_radix_scatter(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks,
+// scoord != nullptr: also write sorted coordinates and full cell keys (last
+// pass).  staged: LDS-staged variant (one wave per chunk, coalesced stores).
+void launch_radix_scatter(hipStream_t s, bool wide, bool staged, const DJob* jobs, const uint2* chunks,
                           const uint32_t* chunk_first, uint32_t n_chunks, const void* kin,
                           const uint32_t* iin, void* kout, uint32_t* iout, int shift, const uint32_t* offs,
                           const float* X, float* scoord);
+// Two-stage sort, stage 2 input: job order from its base's sorted points
+// (bidx + bbase[job]); key = box key of dims 1.. (or, with scoord, the full
+// key plus the sorted coordinates).
+void launch_major_keys(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
+                       const uint64_t* bbase, const uint32_t* bidx, const float* X, void* keys,
+                       uint32_t* idx, float* scoord);
 uint64_t scan_scratch_elems(uint64_t m);
 void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t m,
                            uint32_t* scratch);

@@ -76,6 +76,14 @@ int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h);
 int ent_ctx_last_stats(ent_ctx* ctx, double* density_ms, double* total_ms, uint64_t* pairs,
                        uint64_t* tiles);
 
+/* Sort shape of the last cell-list run: out[0] = radix passes launched (over
+ * the shared dim-0 sorts plus over the jobs), out[1] = shared dim-0 sorts
+ * (0 when every job was sorted by its full cell key).  Jobs with the same
+ * dim-0 window and bandwidth share one dim-0 sort when that halves the
+ * points to sort; ENT_SORT=full|shared in the environment forces a mode.
+ * No reference counterpart. */
+int ent_ctx_last_sort(ent_ctx* ctx, uint32_t* out2);
+
 /* Capture the post-planning launch sequence (cell keys, radix passes, gather,
  * tiles) and the persistent density launch in hipGraphs that are replayed
  * while the plan shape repeats (default on); 0 launches eagerly.  No

@@ -160,10 +160,16 @@ def test_bin_edge_cases(ctx, orc):
         np.testing.assert_array_equal(g[0], o[0])
 
 
-def test_cell_sort_bit_exact(ctx, orc):
-    """GPU stable cell sort == CPU stable counting sort by the same keys."""
+@pytest.mark.parametrize("sort_mode,scatter", [("auto", "staged"), ("shared", "staged"), ("full", "direct")])
+def test_cell_sort_bit_exact(ctx, orc, monkeypatch, sort_mode, scatter):
+    """GPU stable cell sort == CPU stable counting sort by the same keys, for
+    the one-sort-per-job and the shared dim-0 sort schemes and both scatter
+    kernels."""
     from entropical_amd import cell_sort
 
+    if sort_mode != "auto":
+        monkeypatch.setenv("ENT_SORT", sort_mode)
+    monkeypatch.setenv("ENT_SCATTER", scatter)
     r = np.random.default_rng(9)
     n = 30000
     x = r.standard_normal((3, n)).astype(np.float32)

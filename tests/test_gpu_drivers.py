@@ -118,3 +118,70 @@ def test_batch_entropy_uneven_jobs(ctx, orc):
         p = orc.density_fp64(blk, blk.shape[1], list(range(d)), [hh] * d)
         ref = orc.entropy(p)
         assert abs(got[k] - ref) <= RTOL * abs(ref), (k, got[k], ref)
+
+
+def _shared_window_batch(c, n, lags):
+    """MI-style 1-D/2-D jobs plus TE-style lagged 3-D jobs over c columns of
+    n rows (column k at offset k*n): many jobs share a dim-0 window."""
+    specs, out = [], 0
+    for i in range(c):
+        specs.append({"n_dim": 1, "n": n, "n_rows": n, "off": [i * n], "h": [0.2 + 0.05 * i]})
+        for j in range(i + 1, c):
+            specs.append({"n_dim": 2, "n": n, "n_rows": n, "off": [i * n, j * n], "h": [0.2 + 0.05 * i, 0.3]})
+    for tau in lags:
+        m = n - tau
+        for i in range(c):
+            for j in range(c):
+                if i != j:
+                    specs.append({"n_dim": 3, "n": m, "n_rows": n, "off": [i * n, i * n + tau, j * n],
+                                  "h": [0.2 + 0.05 * i, 0.25, 0.35]})
+    for s in specs:
+        s["out_off"] = out
+        out += s["n"]
+    return specs, out
+
+
+def _batch(ctx, ptr, specs, total):
+    from entropical_amd import batch_densities_dev
+
+    dptr = ctx.alloc(8 * total)
+    try:
+        batch_densities_dev(ctx, ptr, specs, dptr)
+        got = np.empty(total, dtype=np.float64)
+        ctx.d2h(got, dptr)
+    finally:
+        ctx.free(dptr)
+    return got
+
+
+@pytest.mark.parametrize("sort_mode", ["full", "shared"])
+@pytest.mark.parametrize("scatter", ["direct", "staged"])
+def test_sort_variants_bit_identical(ctx, orc, monkeypatch, sort_mode, scatter):
+    """One sort per job by its full cell key, or shared dim-0 sorts followed
+    by passes over the remaining dims' key, with either scatter kernel, all
+    give the same permutation: densities are bit-identical to the default
+    run's, and that run matches the FP64 oracle."""
+    c, n = 4, 12000
+    x = coupled(11, c, n)
+    specs, total = _shared_window_batch(c, n, [1, 5])
+    ptr = ctx.alloc(x.nbytes)
+    try:
+        ctx.h2d(ptr, x)
+        ref_bits = _batch(ctx, ptr, specs, total)
+        # auto picks the shared sort here: one base per (column, window length)
+        assert ctx.last_stats()["sort_bases"] == c * 3
+        monkeypatch.setenv("ENT_SORT", sort_mode)
+        monkeypatch.setenv("ENT_SCATTER", scatter)
+        got = _batch(ctx, ptr, specs, total)
+        st = ctx.last_stats()
+    finally:
+        ctx.free(ptr)
+    assert (st["sort_bases"] > 0) == (sort_mode == "shared")
+    np.testing.assert_array_equal(got, ref_bits)
+    for k in (0, 1, len(specs) - 1):
+        s = specs[k]
+        cols = [o // n for o in s["off"]]
+        taus = [o % n for o in s["off"]]
+        p = orc.density_fp64(x, n, cols, s["h"], taus if s["n_dim"] == 3 else None)
+        g = got[s["out_off"]: s["out_off"] + s["n"]]
+        assert np.max(np.abs(g - p) / p) < 1e-5, k
