@@ -9,6 +9,7 @@ from .kde import (  # noqa: F401
     DeviceArray,
     batch_autotune,
     batch_entropy,
+    hestimate,
     mi_autotune,
     te_autotune,
     last_driver_ms,
@@ -40,6 +41,7 @@ __all__ = [
     "te_matrix",
     "batch_entropy",
     "last_driver_ms",
+    "hestimate",
     "mi_autotune",
     "te_autotune",
     "batch_autotune",

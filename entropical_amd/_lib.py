@@ -22,6 +22,8 @@ ENT_ERR_UNSUPPORTED = 6
 
 METHOD_CELL = 0
 METHOD_BRUTE = 1
+HEST_THUMB = 0
+HEST_AMISE = 1
 
 _STATUS_NAMES = {
     ENT_ERR_ARG: "ENT_ERR_ARG",
@@ -87,6 +89,7 @@ SIGNATURES = {
     "ent_batch_entropy_dev": (C.c_int, [P, P, C.POINTER(ent_job), C.c_size_t, C.c_int, P]),
     "ent_ctx_last_driver_ms": (C.c_int, [P, F64P]),
     "ent_batch_loo_dev": (C.c_int, [P, P, C.POINTER(ent_job), C.c_size_t, C.c_double, C.c_int, P]),
+    "ent_hestimate_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, C.c_int, P]),
     "ent_mi_autotune_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P]),
     "ent_te_autotune_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P, C.c_size_t, P]),
     "ent_autotune_dev": (C.c_int, [P, P, C.POINTER(ent_job), C.c_size_t, P, F64P]),

@@ -5,6 +5,7 @@
 // density arrays on the device in FP64; only the per-result scalars return
 // to the host.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -390,6 +391,145 @@ int ent_batch_loo_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, siz
   std::vector<double> res;
   DRV_TRY(run_and_reduce(ctx, X_dev, js, method, items, res));
   for (size_t j = 0; j < n_jobs; ++j) loglik[j] = res[j];
+  return ENT_OK;
+}
+
+namespace {
+
+// hestimate.cpp:42-53 with the self-convolution value supplied: float
+// intermediates exactly where the reference rounds to float.
+float amise_of(float conv, float h, size_t n) {
+  float a = (float)(0.04 * 1.0 / std::pow((double)h, 5.0) * (double)conv);
+  a = (float)std::pow((double)a, 0.2);
+  return (float)((double)a * 1.25 * std::pow(0.6, 0.8) / std::pow((double)(float)n, 0.8));
+}
+
+// One step of AmiseMin::Kernel::_interval_optimized_h (hestimate.cpp:55-86)
+// for one column, unrolled from the reference's recursion into a loop.
+struct AmiseSearch {
+  float prec, lo, hi, pl, ph, h;
+  unsigned iter;
+  bool done;
+  // evaluates the termination test; when not done, sets the two pivots
+  void step() {
+    const float delta = hi - lo;
+    const float pivot = (float)((double)lo + 0.5 * (double)delta);
+    if (iter == 0 || delta <= prec) {
+      h = pivot;
+      done = true;
+      return;
+    }
+    pl = (float)((double)pivot - 0.25 * (double)delta);
+    ph = (float)((double)pivot + 0.25 * (double)delta);
+  }
+  void update(float al, float ah) {
+    const float delta = hi - lo;
+    const float pivot = (float)((double)lo + 0.5 * (double)delta);
+    if (al < ah) hi = pivot;
+    else lo = pivot;
+    --iter;
+  }
+};
+
+}  // namespace
+
+int ent_hestimate_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, int kind, float* h_out) {
+  if (!ctx || !ctx->eng || !X_dev || !h_out || n_rows == 0 || n_cols == 0) {
+    set_error("null argument or empty input");
+    return ENT_ERR_ARG;
+  }
+  if (kind != ENT_HEST_THUMB && kind != ENT_HEST_AMISE) {
+    set_error("unknown bandwidth estimator");
+    return ENT_ERR_ARG;
+  }
+  if (n_rows >= (1ull << 32)) {
+    set_error("more than 2^32 rows");
+    return ENT_ERR_UNSUPPORTED;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  ent::Engine& E = *ctx->eng;
+  DRV_HIP(hipSetDevice(E.device()));
+  // Thumb::Kernel (hestimate.cpp:19-30): N^(-1/7) * sqrt(population variance);
+  // the variance in FP64 on the device (the reference accumulates in float)
+  const uint32_t C = (uint32_t)n_cols;
+  const uint32_t bpc = (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, n_rows / 16384));
+  DRV_TRY(E.ensure(ctx->red, sizeof(double) * ((size_t)C * bpc + 2 * (size_t)C)));
+  double* d_part = (double*)ctx->red.p;
+  double* d_mean = d_part + (size_t)C * bpc;
+  double* d_var = d_mean + C;
+  ent::launch_col_moments(E.stream(), X_dev, n_rows, C, d_part, bpc, d_mean, d_var);
+  DRV_HIP(hipGetLastError());
+  std::vector<double> var(C);
+  DRV_HIP(hipMemcpyAsync(var.data(), d_var, sizeof(double) * C, hipMemcpyDeviceToHost, E.stream()));
+  DRV_HIP(hipStreamSynchronize(E.stream()));
+  std::vector<float> h0(C);
+  for (uint32_t c = 0; c < C; ++c) {
+    h0[c] = (float)(std::pow((double)n_rows, -1.0 / 7.0) * std::sqrt(var[c]));
+    if (!std::isfinite(h0[c])) {
+      set_error("non-finite coordinate in bandwidth estimation input");
+      return ENT_ERR_NONFINITE;
+    }
+  }
+  if (kind == ENT_HEST_THUMB) {
+    std::memcpy(h_out, h0.data(), sizeof(float) * C);
+    ctx->driver_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return ENT_OK;
+  }
+  for (uint32_t c = 0; c < C; ++c)
+    if (!(h0[c] > 0.0f)) {
+      set_error("error: column with zero variance has no AMISE bandwidth");
+      return ENT_ERR_ARG;
+    }
+  // AmiseMin::Kernel (hestimate.cpp:88-104): every column's interval search in
+  // lockstep, both pivots of every unfinished column in one batch per step;
+  // the self-convolution is the engine's triangular-kernel job, as in
+  // ent_epa_convolution: (1/N^2) sum_ij 4.5 max(0, 1 - |x_i - x_j| / (2h)).
+  std::vector<AmiseSearch> S(C);
+  for (uint32_t c = 0; c < C; ++c) {
+    S[c].prec = (float)(0.01 * (double)h0[c]);
+    S[c].lo = (float)(0.1 * (double)h0[c]);
+    S[c].hi = 2.0f * h0[c];
+    S[c].iter = 20;
+    S[c].done = false;
+  }
+  const double norm = 4.5 / ((double)n_rows * (double)n_rows);
+  for (;;) {
+    std::vector<uint32_t> act;
+    for (uint32_t c = 0; c < C; ++c)
+      if (!S[c].done) {
+        S[c].step();
+        if (!S[c].done) act.push_back(c);
+      }
+    if (act.empty()) break;
+    std::vector<ent::JobSpec> js;
+    std::vector<ent::RedItem> items;
+    uint64_t next = 0;
+    for (uint32_t c : act)
+      for (int side = 0; side < 2; ++side) {
+        ent::JobSpec j;
+        j.d = 1;
+        j.n = (uint32_t)n_rows;
+        j.n_rows = (uint32_t)n_rows;
+        j.off[0] = (uint64_t)c * n_rows;
+        j.h[0] = 2.0f * (side ? S[c].ph : S[c].pl);
+        j.flags = ent::kJobTri;
+        j.norm = norm;
+        j.out_off = next;
+        js.push_back(j);
+        items.push_back(item(ent::kRedSum, {next}, n_rows, 0.0, 1.0));
+        next += n_rows;
+      }
+    std::vector<double> conv;
+    DRV_TRY(run_and_reduce(ctx, X_dev, js, ent::kMethodCell, items, conv));
+    for (size_t a = 0; a < act.size(); ++a) {
+      AmiseSearch& s = S[act[a]];
+      const float al = amise_of((float)conv[2 * a], s.pl, n_rows);
+      const float ah = amise_of((float)conv[2 * a + 1], s.ph, n_rows);
+      s.update(al, ah);
+    }
+  }
+  for (uint32_t c = 0; c < C; ++c) h_out[c] = S[c].h;
+  ctx->driver_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return ENT_OK;
 }
 

@@ -43,6 +43,8 @@ __device__ __forceinline__ double term(const RedItem& it, uint64_t k, const doub
       const double loo = (nd * a - it.pref) / (nd - 1.0);
       return log(loo > it.scale ? loo : it.scale);
     }
+    case kRedSum:
+      return a * it.scale;
     default:
       return 0.0;
   }
@@ -79,7 +81,57 @@ __global__ void k_reduce_final(const double* __restrict__ partial, uint32_t bloc
   out[item] = acc;
 }
 
+// Column sums (mean == nullptr) or centred square sums of X, one partial per
+// (column, block); blockIdx.y = column.
+__global__ __launch_bounds__(kRBlock) void k_col_partial(const float* __restrict__ X, uint64_t n_rows,
+                                                         const double* __restrict__ mean,
+                                                         double* __restrict__ partial,
+                                                         uint32_t blocks_per_col) {
+  __shared__ double s[kRBlock];
+  const uint32_t col = blockIdx.y;
+  const float* x = X + (uint64_t)col * n_rows;
+  const uint64_t per = (n_rows + blocks_per_col - 1) / blocks_per_col;
+  const uint64_t k0 = (uint64_t)blockIdx.x * per;
+  const uint64_t k1 = k0 + per < n_rows ? k0 + per : n_rows;
+  const double m = mean ? mean[col] : 0.0;
+  double acc = 0.0;
+  for (uint64_t k = k0 + threadIdx.x; k < k1; k += kRBlock) {
+    const double v = (double)x[k] - m;
+    acc += mean ? v * v : v;
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = kRBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(uint64_t)col * blocks_per_col + blockIdx.x] = s[0];
+}
+
+__global__ void k_col_final(const double* __restrict__ partial, uint32_t blocks_per_col, uint32_t n_cols,
+                            double inv_n, double* __restrict__ out) {
+  const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n_cols) return;
+  double acc = 0.0;
+  for (uint32_t b = 0; b < blocks_per_col; ++b) acc += partial[(uint64_t)col * blocks_per_col + b];
+  out[col] = acc * inv_n;
+}
+
 }  // namespace
+
+void launch_col_moments(hipStream_t s, const float* X, uint64_t n_rows, uint32_t n_cols, double* partial,
+                        uint32_t blocks_per_col, double* mean, double* var) {
+  if (n_cols == 0 || n_rows == 0) return;
+  const double inv_n = 1.0 / (double)n_rows;
+  const dim3 grid(blocks_per_col, n_cols);
+  const dim3 fin((n_cols + 255) / 256);
+  hipLaunchKernelGGL(k_col_partial, grid, dim3(kRBlock), 0, s, X, n_rows, (const double*)nullptr, partial,
+                     blocks_per_col);
+  hipLaunchKernelGGL(k_col_final, fin, dim3(256), 0, s, partial, blocks_per_col, n_cols, inv_n, mean);
+  hipLaunchKernelGGL(k_col_partial, grid, dim3(kRBlock), 0, s, X, n_rows, (const double*)mean, partial,
+                     blocks_per_col);
+  hipLaunchKernelGGL(k_col_final, fin, dim3(256), 0, s, partial, blocks_per_col, n_cols, inv_n, var);
+}
 
 void launch_reduce(hipStream_t s, const RedItem* items, uint32_t n_items, const double* D,
                    double* partial, uint32_t blocks_per_item, double* out) {

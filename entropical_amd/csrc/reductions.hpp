@@ -13,6 +13,7 @@ enum RedOp : uint32_t {
   kRedLoo = 4,      // sum ln max(p_{-i}, floor), p_{-i} = (n p - K0) / (n - 1):
                     // leave-one-out log-likelihood (p includes the self pair
                     // K0 = prod 0.75/h); pref = K0, scale = floor density
+  kRedSum = 5,      // sum p (epa self-convolution rows, densities_omp.cpp:271-276)
 };
 
 // off[0..3] index density arrays in one device buffer D (doubles).
@@ -27,5 +28,12 @@ struct RedItem {
 
 void launch_reduce(hipStream_t s, const RedItem* items, uint32_t n_items, const double* D,
                    double* partial, uint32_t blocks_per_item, double* out);
+
+// Per column of a column-major float32 array X[col*n_rows + row]: mean and
+// population variance in FP64 (two passes: sum, then centred squares), with
+// a fixed partition so results are deterministic.  partial holds
+// n_cols*blocks_per_col doubles; mean and var n_cols each.
+void launch_col_moments(hipStream_t s, const float* X, uint64_t n_rows, uint32_t n_cols, double* partial,
+                        uint32_t blocks_per_col, double* mean, double* var);
 
 }  // namespace ent

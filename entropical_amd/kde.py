@@ -293,6 +293,19 @@ def te_matrix(ctx: Context, x: DeviceArray, h: Sequence[float], taus: Sequence[i
     return out
 
 
+def hestimate(ctx: Context, x: DeviceArray, kind: str = "amise") -> np.ndarray:
+    """Bandwidth per column (Hestimate::main, hestimate.hxx:6-24): "thumb"
+    (hestimate.cpp:19-30) or "amise" (hestimate.cpp:41-104, all columns'
+    interval searches batched on the device)."""
+    kinds = {"thumb": _lib.HEST_THUMB, "amise": _lib.HEST_AMISE}
+    if kind not in kinds:
+        raise ValueError(f"unknown bandwidth estimator {kind!r}")
+    n_cols, n_rows = x.shape
+    out = np.empty(n_cols, dtype=np.float32)
+    check(load().ent_hestimate_dev(ctx.handle, C.c_void_p(x.ptr), n_rows, n_cols, kinds[kind], _ptr(out)))
+    return out
+
+
 def make_jobs(specs: Sequence[dict]):
     arr = (_lib.ent_job * len(specs))()
     for a, j in zip(arr, specs):

@@ -9,9 +9,11 @@ log-likelihood argmax over 32 candidate multipliers, computed by the GPU KDE
 (``ent_batch_loo_dev``).  The rule of thumb (and AMISE through
 ``epa_convolution``) remain the fallback and the yardstick.
 
-Training data: seeded synthetic samples with d in 1..3,
-N log-uniform in [1000, 50000], 1-5 Gaussian or Student-t components with
-random covariances.
+Training data: seeded synthetic samples with d in 1..3, N log-uniform in
+[1000, 200000], 1-5 Gaussian or Student-t components with random
+covariances.  The per-feature range of the training set is stored with the
+weights; inputs outside it get the AMISE bandwidth (hestimate) instead of the
+network's.
 """
 from __future__ import annotations
 
@@ -36,7 +38,10 @@ WEIGHTS = Path(__file__).resolve().parent / "learned_h_weights.npz"
 
 
 # ----------------------------------------------------------------- data
-def synth_dataset(rng: np.random.Generator, n_min: int = 1000, n_max: int = 50000) -> np.ndarray:
+N_MIN, N_MAX = 1000, 200_000
+
+
+def synth_dataset(rng: np.random.Generator, n_min: int = N_MIN, n_max: int = N_MAX) -> np.ndarray:
     """One training sample: (d, N) float32 points from a random mixture."""
     d = int(rng.integers(1, 4))
     n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
@@ -112,6 +117,15 @@ def features(x: np.ndarray) -> np.ndarray:
 
 
 # --------------------------------------------------------------- labels
+def batch_loo(ctx: Context, x_dev_ptr: int, specs: Sequence[dict], floor_rel: float = FLOOR_REL) -> np.ndarray:
+    """Leave-one-out log-likelihood (sum over points) of every job, GPU KDE."""
+    arr = make_jobs(specs)
+    res = np.empty(len(specs))
+    check(load().ent_batch_loo_dev(ctx.handle, C.c_void_p(x_dev_ptr), arr, len(specs), float(floor_rel),
+                                   _lib.METHOD_CELL, _ptr(res)))
+    return res
+
+
 def loo_grid(ctx: Context, datasets: Sequence[np.ndarray], r_grid=R_GRID, floor_rel: float = FLOOR_REL,
              max_points: int = 400_000_000) -> np.ndarray:
     """LOO log-likelihood (GPU KDE) of every dataset at h = r * thumb for each
@@ -138,10 +152,7 @@ def loo_grid(ctx: Context, datasets: Sequence[np.ndarray], r_grid=R_GRID, floor_
                     specs.append({"n_dim": d, "n": n, "n_rows": n, "off": [off + e * n for e in range(d)],
                                   "h": [float(r * th[e]) for e in range(d)]})
                 off += d * n
-            arr = make_jobs(specs)
-            res = np.empty(len(specs))
-            check(load().ent_batch_loo_dev(ctx.handle, C.c_void_p(dptr), arr, len(specs), float(floor_rel),
-                                           _lib.METHOD_CELL, _ptr(res)))
+            res = batch_loo(ctx, dptr, specs, floor_rel)
         finally:
             ctx.free(dptr)
         out[i:j] = res.reshape(len(group), len(r_grid))
@@ -177,6 +188,9 @@ class BandwidthModel:
         self.n_params = n.value
         self.mean = np.zeros(N_FEATURES, np.float32)
         self.std = np.ones(N_FEATURES, np.float32)
+        # per-feature range of the training set (the network's envelope)
+        self.fmin = np.full(N_FEATURES, -np.inf, np.float32)
+        self.fmax = np.full(N_FEATURES, np.inf, np.float32)
 
     def close(self) -> None:
         if self._h is not None:
@@ -201,8 +215,8 @@ class BandwidthModel:
         check(load().ent_mlp_set_params(self.ctx.handle, self._h, _ptr(p)))
 
     def save(self, path: Path, meta: Optional[dict] = None) -> None:
-        np.savez_compressed(path, params=self.get_params(), mean=self.mean, std=self.std,
-                            meta=np.array(json.dumps(meta or {})))
+        np.savez_compressed(path, params=self.get_params(), mean=self.mean, std=self.std, fmin=self.fmin,
+                            fmax=self.fmax, meta=np.array(json.dumps(meta or {})))
 
     @classmethod
     def load(cls, ctx: Context, path: Path = WEIGHTS) -> "BandwidthModel":
@@ -211,6 +225,9 @@ class BandwidthModel:
         m.set_params(z["params"])
         m.mean = z["mean"].astype(np.float32)
         m.std = z["std"].astype(np.float32)
+        if "fmin" in z.files:
+            m.fmin = z["fmin"].astype(np.float32)
+            m.fmax = z["fmax"].astype(np.float32)
         return m
 
     # compute -----------------------------------------------------------
@@ -246,6 +263,8 @@ class BandwidthModel:
     def fit(self, F: np.ndarray, t: np.ndarray, epochs: int = 200, batch: int = 256, lr: float = 1e-3,
             seed: int = 0) -> np.ndarray:
         F = np.asarray(F, np.float32)
+        self.fmin = F.min(axis=0)
+        self.fmax = F.max(axis=0)
         self.mean = F.mean(axis=0)
         self.std = F.std(axis=0)
         self.std[self.std < 1e-6] = 1.0
@@ -256,9 +275,33 @@ class BandwidthModel:
         lo, hi = np.log2(R_GRID[0]), np.log2(R_GRID[-1])
         return np.clip(lr, lo, hi)
 
+    def in_envelope(self, F: np.ndarray, margin: float = 0.05) -> np.ndarray:
+        """True per row of F whose features all lie within the training range
+        (widened by `margin` of each feature's span)."""
+        F = np.atleast_2d(np.asarray(F, np.float32))
+        span = self.fmax - self.fmin
+        span = np.where(np.isfinite(span), span, 0.0)
+        return np.all((F >= self.fmin - margin * span) & (F <= self.fmax + margin * span), axis=1)
+
     def bandwidth(self, x: np.ndarray) -> np.ndarray:
-        """Per-dimension bandwidth for a (d, N) sample."""
+        """Per-dimension bandwidth for a (d, N) sample (no envelope check)."""
         return thumb(x) * 2.0 ** float(self.predict_log2r(features(x))[0])
+
+
+def column_bandwidths(ctx: Context, model: BandwidthModel, x_dev, X: np.ndarray):
+    """Learned bandwidth of every column of X (n_cols, n_rows), device copy
+    x_dev: h = r * thumb with the network's r where the column's features lie
+    in the training envelope, else the AMISE bandwidth (hestimate, computed
+    on the device).  Returns (h float32 (n_cols,), used_network bool)."""
+    from .kde import hestimate
+
+    F = np.stack([features(X[c: c + 1]) for c in range(X.shape[0])])
+    th = np.array([thumb(X[c: c + 1])[0] for c in range(X.shape[0])])
+    ok = model.in_envelope(F)
+    h = (th * 2.0 ** model.predict_log2r(F)).astype(np.float32)
+    if not ok.all():
+        h[~ok] = hestimate(ctx, x_dev, "amise")[~ok]
+    return h, ok
 
 
 # -------------------------------------------------------------- training
@@ -282,6 +325,63 @@ def make_training_set(ctx: Context, n_datasets: int, seed: int, log=print):
     return F, Y, LL, dims
 
 
+def evaluate_mi(ctx: Context, model: BandwidthModel, n_cols: int = 64, n_rows: int = 200_000, log=print) -> dict:
+    """The learned bandwidth on the MI workload's columns (64 x 200,000
+    mixed columns, datasets.mixed_columns) against hestimate: per column the
+    rule of thumb, the AMISE bandwidth and the network's choice (with its
+    AMISE fallback); each judged by the entropy it gives relative to the
+    AMISE bandwidth's entropy, by its leave-one-out log-likelihood per point
+    against the best of the 32-multiplier grid, and by the MI matrix it
+    gives.  All densities on the GPU."""
+    from . import datasets
+    from .kde import DeviceArray, batch_entropy, hestimate, mi_matrix
+
+    X = datasets.mixed_columns(n_cols, n_rows)
+    dx = DeviceArray(ctx, X)
+    try:
+        t0 = time.perf_counter()
+        h_thumb = hestimate(ctx, dx, "thumb")
+        t_thumb = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        h_amise = hestimate(ctx, dx, "amise")
+        t_amise = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        h_mlp, used = column_bandwidths(ctx, model, dx, X)
+        t_mlp = time.perf_counter() - t0
+        grid = loo_grid(ctx, [X[c: c + 1] for c in range(n_cols)]) / n_rows  # (n_cols, 32) per point
+        best = grid.max(axis=1)
+        out = {"n_cols": n_cols, "n_rows": n_rows, "network_used_columns": int(used.sum()),
+               "seconds": {"thumb": t_thumb, "amise": t_amise, "network": t_mlp}}
+        H, LL, MI = {}, {}, {}
+        for name, h in (("thumb", h_thumb), ("amise", h_amise), ("network", h_mlp)):
+            specs = [{"n_dim": 1, "n": n_rows, "n_rows": n_rows, "off": [c * n_rows], "h": [float(h[c])]}
+                     for c in range(n_cols)]
+            H[name] = batch_entropy(ctx, dx.ptr, specs)
+            LL[name] = batch_loo(ctx, dx.ptr, specs) / n_rows
+            MI[name] = mi_matrix(ctx, dx, [float(v) for v in h])
+        iu = np.triu_indices(n_cols, 1)
+        for name in ("thumb", "network"):
+            rel = np.abs(H[name] - H["amise"]) / np.abs(H["amise"])
+            dmi = np.abs(MI[name][iu] - MI["amise"][iu])
+            out[name] = {
+                "entropy_rel_err_vs_amise_mean": float(rel.mean()),
+                "entropy_rel_err_vs_amise_max": float(rel.max()),
+                "mi_abs_diff_vs_amise_mean": float(dmi.mean()),
+                "mi_rel_diff_vs_amise_mean": float(dmi.mean() / np.abs(MI["amise"][iu]).mean()),
+                "loo_per_point_mean": float(LL[name].mean()),
+                "loo_gap_to_grid_best_mean": float((best - LL[name]).mean()),
+                "frac_columns_loo_above_amise": float(np.mean(LL[name] > LL["amise"])),
+                "h_over_amise_median": float(np.median(h_thumb / h_amise if name == "thumb" else h_mlp / h_amise)),
+            }
+        out["amise"] = {"loo_per_point_mean": float(LL["amise"].mean()),
+                        "loo_gap_to_grid_best_mean": float((best - LL["amise"]).mean())}
+        out["grid_best_loo_per_point_mean"] = float(best.mean())
+        log(f"[learned_h] MI workload: {json.dumps(out)}")
+        return out
+    finally:
+        dx.free()
+
+
 def main(argv=None) -> int:
     import argparse
 
@@ -292,9 +392,22 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=0xE17A0005)
     ap.add_argument("--out", type=Path, default=WEIGHTS)
     ap.add_argument("--report", type=Path, default=None)
+    ap.add_argument("--eval-mi-only", action="store_true",
+                    help="skip training; evaluate the weights at --out on the MI workload")
+    ap.add_argument("--no-eval-mi", action="store_true")
     a = ap.parse_args(argv)
     ctx = Context(0)
+    if a.eval_mi_only:
+        model = BandwidthModel.load(ctx, a.out)
+        rep = json.loads(a.report.read_text()) if a.report and a.report.exists() else {}
+        rep["mi_workload"] = evaluate_mi(ctx, model)
+        print(json.dumps(rep["mi_workload"]), flush=True)
+        if a.report:
+            a.report.write_text(json.dumps(rep, indent=1))
+        return 0
+    t0 = time.time()
     F, Y, LL, dims = make_training_set(ctx, a.datasets + a.holdout, a.seed)
+    label_s = time.time() - t0
     ntr = a.datasets
     model = BandwidthModel(ctx, seed=1)
     t0 = time.time()
@@ -306,17 +419,23 @@ def main(argv=None) -> int:
     gain_mlp = np.array([ll_at(LL[ntr + k], pred[k]) for k in range(a.holdout)])
     gain_thumb = np.array([ll_at(LL[ntr + k], 0.0) for k in range(a.holdout)])
     best = LL[ntr:].max(axis=1)
+    n_pts = 10.0 ** F[ntr:, 0].astype(np.float64)
     rep = {
-        "datasets": a.datasets, "holdout": a.holdout, "epochs": a.epochs, "train_s": train_s,
+        "datasets": a.datasets, "holdout": a.holdout, "epochs": a.epochs, "n_range": [N_MIN, N_MAX],
+        "label_s": label_s, "train_s": train_s,
         "final_train_mse": float(losses[-1]), "first_train_mse": float(losses[0]),
         "holdout_mae_log2r_mlp": float(np.mean(np.abs(pred - Y[ntr:]))),
         "holdout_mae_log2r_thumb": float(np.mean(np.abs(Y[ntr:]))),
         "holdout_mean_loo_gap_mlp": float(np.mean(best - gain_mlp)),
         "holdout_mean_loo_gap_thumb": float(np.mean(best - gain_thumb)),
+        "holdout_mean_loo_gap_per_point_mlp": float(np.mean((best - gain_mlp) / n_pts)),
+        "holdout_mean_loo_gap_per_point_thumb": float(np.mean((best - gain_thumb) / n_pts)),
         "holdout_frac_mlp_better": float(np.mean(gain_mlp > gain_thumb)),
     }
     model.save(a.out, meta=rep)
     print(json.dumps(rep), flush=True)
+    if not a.no_eval_mi:
+        rep["mi_workload"] = evaluate_mi(ctx, model)
     if a.report:
         a.report.write_text(json.dumps(rep, indent=1))
     return 0

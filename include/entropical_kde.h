@@ -33,6 +33,9 @@ extern "C" {
 #define ENT_METHOD_CELL 0     /* cell-list candidates (default) */
 #define ENT_METHOD_BRUTE 1    /* all pairs, as the reference's CUDA backend does */
 
+#define ENT_HEST_THUMB 0      /* Hestimate::Thumb (hestimate.cpp:19-30) */
+#define ENT_HEST_AMISE 1      /* Hestimate::AmiseMin (hestimate.cpp:41-104) */
+
 typedef struct ent_ctx ent_ctx;
 
 /* One density estimate in a batch.  Point k (0 <= k < n), dim e:
@@ -181,6 +184,17 @@ int ent_batch_entropy_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs,
  * i.e. floor_rel of one coincident neighbour.  jobs[j].n_rows is ignored. */
 int ent_batch_loo_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs, double floor_rel,
                       int method, double* loglik);
+
+/* Bandwidth per column of device-resident column-major X, as Hestimate::main
+ * computes it for every selected column (hestimate.hxx:6-24):
+ * ENT_HEST_THUMB: N^(-1/7) * population sd (Thumb::Kernel,
+ * hestimate.cpp:19-30; variance in FP64);
+ * ENT_HEST_AMISE: AmiseMin::Kernel (hestimate.cpp:41-104): interval search
+ * from [0.1 h0, 2 h0] to 0.01 h0 (at most 20 halvings) for the minimum of the
+ * AMISE built on epa_convolution.  All columns search in lockstep: each step
+ * evaluates both pivots of every unfinished column in one batched launch.
+ * Zero-variance columns are an error for AMISE. */
+int ent_hestimate_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, int kind, float* h_out);
 
 /* Tile-shape autotuner (no reference counterpart; the reference fixes its
  * tile at compile time, densities_cuda.cu:14).  Times candidate settings of

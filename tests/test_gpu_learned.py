@@ -96,3 +96,33 @@ def test_trained_network_beats_rule_of_thumb(ctx):
     assert gain > 0.0
     assert np.median(np.abs(pred - best)) < np.median(np.abs(best))
     model.close()
+
+
+def test_network_on_mi_columns_against_hestimate(ctx):
+    """MI-workload columns (seeded mixed columns, reduced to 8 x 50,000): the
+    network's bandwidths lie in its training envelope, and their leave-one-out
+    log-likelihood is closer to the best of the 32-multiplier grid than both
+    hestimate bandwidths' (rule of thumb and AMISE, computed on the GPU)."""
+    from entropical_amd import DeviceArray, datasets, hestimate
+    from entropical_amd import learned_h as L
+
+    X = datasets.mixed_columns(8, 50_000)
+    n = X.shape[1]
+    model = L.BandwidthModel.load(ctx)
+    dx = DeviceArray(ctx, X)
+    try:
+        h_net, used = L.column_bandwidths(ctx, model, dx, X)
+        assert used.all()
+        h_th = hestimate(ctx, dx, "thumb")
+        h_am = hestimate(ctx, dx, "amise")
+        best = L.loo_grid(ctx, [X[c: c + 1] for c in range(8)]).max(axis=1) / n
+
+        def loo(h):
+            specs = [{"n_dim": 1, "n": n, "n_rows": n, "off": [c * n], "h": [float(h[c])]} for c in range(8)]
+            return L.batch_loo(ctx, dx.ptr, specs) / n
+
+        gap = {k: float(np.mean(best - loo(h))) for k, h in (("net", h_net), ("thumb", h_th), ("amise", h_am))}
+    finally:
+        dx.free()
+        model.close()
+    assert gap["net"] < gap["thumb"] and gap["net"] < gap["amise"], gap
