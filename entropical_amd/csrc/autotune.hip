@@ -2,9 +2,9 @@
 //
 // Search space: maximum queries per lane q (tile = 64*q queries), dim-0 boxes
 // per bandwidth m0, row boxes per bandwidth mr, tile extent along dim 0 in
-// bandwidths group_h.  Coordinate descent from the current setting, two
-// sweeps, each candidate timed on a subsample of the batch (second of two
-// runs, so graph capture and workspace growth are excluded).
+// bandwidths group_h.  Every setting of the grid is timed on a subsample of
+// the batch (after one untimed run, so graph capture and workspace growth are
+// excluded); the four fastest are timed again three times each.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
@@ -74,42 +74,49 @@ int Engine::autotune(const float* X, const std::vector<JobSpec>& jobs, double* d
     return kOk;
   }
 
-  auto eval = [&](const Tuning& t, double* ms) -> int {
+  // time of `timed` runs after one untimed run (graph capture, workspace
+  // growth), best of the timed runs
+  auto eval = [&](const Tuning& t, int timed, double* ms) -> int {
     tuning = t;
     double best = 1e300;
-    for (int rep = 0; rep < 2; ++rep) {
+    for (int rep = 0; rep <= timed; ++rep) {
       int rc = densities(X, sample, dens, kMethodCell);
       if (rc != kOk) return rc;
-      if (rep == 1) best = std::min(best, last.total_ms);
+      if (rep > 0) best = std::min(best, last.total_ms);
     }
     *ms = best;
     return kOk;
   };
+  // Full grid (the axes interact: q bounds a tile's size, m0 * group_h its
+  // extent along dim 0, mr the row height), then the four fastest re-timed.
   const std::vector<int> qs = {1, 2, 4}, m0s = {4, 8, 16}, mrs = {1, 2}, ghs = {2, 4, 8};
-  Tuning cur = tuning;
-  double cur_ms = 0.0;
-  int rc = eval(cur, &cur_ms);
-  if (rc != kOk) return rc;
-  for (int sweep = 0; sweep < 2; ++sweep) {
-    bool improved = false;
-    for (int axis = 0; axis < 4; ++axis) {
-      const std::vector<int>& vals = axis == 0 ? qs : axis == 1 ? m0s : axis == 2 ? mrs : ghs;
-      for (int v : vals) {
-        Tuning t = cur;
-        int* field = axis == 0 ? &t.q : axis == 1 ? &t.m0 : axis == 2 ? &t.mr : &t.group_h;
-        if (*field == v) continue;
-        *field = v;
-        double ms = 0.0;
-        rc = eval(t, &ms);
-        if (rc != kOk) return rc;
-        if (ms < cur_ms * 0.99) {
-          cur = t;
-          cur_ms = ms;
-          improved = true;
+  std::vector<std::pair<double, Tuning>> scored;
+  for (int q : qs)
+    for (int m0 : m0s)
+      for (int mr : mrs)
+        for (int gh : ghs) {
+          Tuning t;
+          t.q = q;
+          t.m0 = m0;
+          t.mr = mr;
+          t.group_h = gh;
+          double ms = 0.0;
+          int rc = eval(t, 1, &ms);
+          if (rc != kOk) return rc;
+          scored.emplace_back(ms, t);
         }
-      }
+  std::sort(scored.begin(), scored.end(),
+            [](const std::pair<double, Tuning>& a, const std::pair<double, Tuning>& b) { return a.first < b.first; });
+  Tuning cur = scored[0].second;
+  double cur_ms = 1e300;
+  for (size_t k = 0; k < std::min<size_t>(4, scored.size()); ++k) {
+    double ms = 0.0;
+    int rc = eval(scored[k].second, 3, &ms);
+    if (rc != kOk) return rc;
+    if (ms < cur_ms) {
+      cur_ms = ms;
+      cur = scored[k].second;
     }
-    if (!improved) break;
   }
   tuning = cur;
   tune_cache_[sig] = cur;

@@ -90,7 +90,7 @@ class Engine {
   Tuning tuning;
   RunStats last;
 
-  // Autotuner: coordinate descent over (q, m0, mr, group_h) on a subsample of
+  // Autotuner: full grid over (q, m0, mr, group_h) on a subsample of
   // the jobs, timing whole runs (planning round trip, sort, tiles, density
   // launch) with HIP events.  Sets `tuning` to the best and caches it under a
   // signature of the workload (in memory, and in the file named by

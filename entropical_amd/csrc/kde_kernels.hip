@@ -633,9 +633,6 @@ __device__ __forceinline__ uint32_t lower_bound_key(const KeyT* a, uint32_t n, u
 
 struct WaveLds {
   float4 cand[kWave];        // staged candidates (local coordinates)
-  uint32_t r_lo[kMaxRanges]; // candidate range start (sorted position)
-  uint32_t r_ps[kMaxRanges]; // flattened prefix start
-  uint32_t r_pe[kMaxRanges]; // flattened prefix end
 };
 
 // Conservative box range [lo, hi] covering [a - h, b + h] along one dim.
@@ -696,21 +693,38 @@ __device__ __forceinline__ void accumulate_chunk(const WaveLds& L, const float (
       } else {
         cs[0][0] = v.x; cs[0][1] = v.y; cs[0][2] = v.z;
       }
-#pragma unroll
-      for (int c = 0; c < kPer4; ++c) {
+      if constexpr (D == 1) {
+        // four candidates per read: the terms are summed as a small tree so
+        // the accumulator sees one dependent add per four candidates (a
+        // serial chain sub -> fma -> add per candidate leaves the SIMD idle
+        // between dependent instructions)
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-          float w[3];
+          float w4[4];
 #pragma unroll
-          for (int e = 0; e < D; ++e) {
-            float uu = sq[e][q] - cs[c][e];
-            if constexpr (RAW) uu = uu * hinv[e];
-            if constexpr (TRI) w[e] = clamp01(1.0f - fabsf(uu));
-            else w[e] = clamp01(__builtin_fmaf(-uu, uu, 1.0f));
+          for (int c = 0; c < 4; ++c) {
+            float uu = sq[0][q] - cs[c][0];
+            if constexpr (RAW) uu = uu * hinv[0];
+            if constexpr (TRI) w4[c] = clamp01(1.0f - fabsf(uu));
+            else w4[c] = clamp01(__builtin_fmaf(-uu, uu, 1.0f));
           }
-          if constexpr (D == 1) part[q] += w[0];
-          else if constexpr (D == 2) part[q] = __builtin_fmaf(w[0], w[1], part[q]);
-          else part[q] = __builtin_fmaf(w[0] * w[1], w[2], part[q]);
+          part[q] += (w4[0] + w4[1]) + (w4[2] + w4[3]);
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < kPer4; ++c) {
+#pragma unroll
+          for (int q = 0; q < Q; ++q) {
+            float w[3];
+#pragma unroll
+            for (int e = 0; e < D; ++e) {
+              float uu = sq[e][q] - cs[c][e];
+              if constexpr (RAW) uu = uu * hinv[e];
+              w[e] = clamp01(__builtin_fmaf(-uu, uu, 1.0f));
+            }
+            if constexpr (D == 2) part[q] = __builtin_fmaf(w[0], w[1], part[q]);
+            else part[q] = __builtin_fmaf(w[0] * w[1], w[2], part[q]);
+          }
         }
       }
       // compensated merge every 32 candidates
@@ -816,15 +830,11 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
     }
   }
 
-  // flattened candidate index space over the ranges
-  uint32_t incl = wave_incl_scan(my_len);
+  // flattened candidate index space over the ranges: lane k < n_ranges
+  // holds range k as (first sorted position, first flattened index)
+  const uint32_t incl = wave_incl_scan(my_len);
   const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, kWave - 1, kWave));
-  if ((uint32_t)l < n_ranges) {
-    L.r_lo[l] = my_lo;
-    L.r_ps[l] = incl - my_len;
-    L.r_pe[l] = incl;
-  }
-  wave_sync();
+  const uint32_t my_ps = incl - my_len;
 
   float sq[3][Q];
 #pragma unroll
@@ -836,37 +846,36 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
 #pragma unroll
   for (int q = 0; q < Q; ++q) { S[q] = 0.0f; C[q] = 0.0f; }
 
-  // candidate loader: flattened index f -> staged local coordinates
-  auto load_cand = [&](uint32_t f, float* s) {
+  // candidate loader: flattened index f -> raw coordinates (converted to
+  // local coordinates only when staged, so the load's latency overlaps the
+  // previous chunk's pair loop).  The range of f is the last range whose
+  // first flattened index is <= f (empty ranges share their successor's
+  // start and are overridden by it); ranges are read from their lanes.
+  auto load_cand = [&](uint32_t f, float* x) {
     if (f < total) {
-      uint32_t r = 0;
-      if (n_ranges > 1) {
-        // first range whose prefix end exceeds f
-        uint32_t lo_r = 0, hi_r = n_ranges - 1;
-        while (lo_r < hi_r) {
-          uint32_t mid = (lo_r + hi_r) >> 1;
-          if (L.r_pe[mid] <= f) lo_r = mid + 1;
-          else hi_r = mid;
-        }
-        r = lo_r;
+      uint32_t off = 0;  // sorted position - flattened index, mod 2^32
+      for (uint32_t k = 0; k < n_ranges; ++k) {
+        const uint32_t ps_k = __builtin_amdgcn_readlane(my_ps, k);
+        const uint32_t lo_k = __builtin_amdgcn_readlane(my_lo, k);
+        if (f >= ps_k) off = lo_k - ps_k;
       }
-      uint32_t p = L.r_lo[r] + (f - L.r_ps[r]);
+      const uint32_t p = f + off;
 #pragma unroll
-      for (int e = 0; e < D; ++e) {
-        float x = src[e][p];
-        s[e] = RAW ? x : (x - o[e]) * hinv[e];
-      }
+      for (int e = 0; e < D; ++e) x[e] = src[e][p];
     } else {
 #pragma unroll
-      for (int e = 0; e < D; ++e) s[e] = kSentinel;
+      for (int e = 0; e < D; ++e) x[e] = kSentinel;
     }
   };
 
   float nxt[3];
   load_cand((uint32_t)l, nxt);
   for (uint32_t base = 0; base < total; base += kWave) {
+    float s_c[3];
+#pragma unroll
+    for (int e = 0; e < D; ++e) s_c[e] = RAW ? nxt[e] : (nxt[e] - o[e]) * hinv[e];
     wave_sync();
-    stage_store<D>(L, l, nxt);
+    stage_store<D>(L, l, s_c);
     wave_sync();
     if (base + kWave < total) load_cand(base + kWave + (uint32_t)l, nxt);
     if constexpr (D == 1) {
@@ -890,7 +899,8 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
 }
 
 // Queries per lane chosen per tile: the smallest Q <= QMAX whose 64*Q lane
-// slots hold the tile, so partly filled tiles do not pay for idle slots.
+// slots hold the tile (Q = 4 steps down through 3, 2 and 1), so partly
+// filled tiles do not pay for idle slots.
 template <int D, int QMAX, bool RAW, typename KeyT>
 __device__ __forceinline__ void run_tile(const DJob& J, const DTile& T, const float* __restrict__ X,
                                          const float* __restrict__ scoord, const KeyT* __restrict__ keys,
@@ -900,8 +910,8 @@ __device__ __forceinline__ void run_tile(const DJob& J, const DTile& T, const fl
   const uint32_t nq = T.p1 - T.p0;
 #ifndef ENT_STATIC_Q
   if constexpr (QMAX >= 2 && !RAW) {  // plain-path tiles are full except the last
-    if (nq <= (uint32_t)(kWave * (QMAX / 2))) {
-      run_tile<D, QMAX / 2, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, pairs, lane_slots);
+    if (nq <= (uint32_t)(kWave * (QMAX - 1))) {
+      run_tile<D, QMAX - 1, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, pairs, lane_slots);
       return;
     }
   }
