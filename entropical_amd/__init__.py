@@ -9,7 +9,10 @@ from .kde import (  # noqa: F401
     DeviceArray,
     batch_autotune,
     batch_entropy,
+    autoent,
     hestimate,
+    kldiv,
+    negs,
     mi_autotune,
     te_autotune,
     last_driver_ms,
@@ -41,7 +44,10 @@ __all__ = [
     "te_matrix",
     "batch_entropy",
     "last_driver_ms",
+    "autoent",
     "hestimate",
+    "kldiv",
+    "negs",
     "mi_autotune",
     "te_autotune",
     "batch_autotune",

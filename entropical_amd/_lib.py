@@ -90,6 +90,9 @@ SIGNATURES = {
     "ent_ctx_last_driver_ms": (C.c_int, [P, F64P]),
     "ent_batch_loo_dev": (C.c_int, [P, P, C.POINTER(ent_job), C.c_size_t, C.c_double, C.c_int, P]),
     "ent_hestimate_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, C.c_int, P]),
+    "ent_autoent_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, C.c_uint32, C.c_int, C.c_int, P]),
+    "ent_kldiv_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, C.c_int, C.c_int, P]),
+    "ent_negs_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, C.c_int, C.c_int, P]),
     "ent_mi_autotune_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P]),
     "ent_te_autotune_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P, C.c_size_t, P]),
     "ent_autotune_dev": (C.c_int, [P, P, C.POINTER(ent_job), C.c_size_t, P, F64P]),

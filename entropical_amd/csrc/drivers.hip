@@ -75,15 +75,10 @@ ent::JobSpec make_job(size_t n_rows, std::initializer_list<uint32_t> cols, std::
   return j;
 }
 
-// Densities of all jobs into ctx->out, then the reduction items; results
-// (one double per item) are copied to host `res`.
-int run_and_reduce(ent_ctx* ctx, const float* X, const std::vector<ent::JobSpec>& jobs, int method,
-                   const std::vector<ent::RedItem>& items, std::vector<double>& res) {
+// Reduction items over the densities already in ctx->out; results (one
+// double per item) are copied to host `res`.
+int reduce_only(ent_ctx* ctx, const std::vector<ent::RedItem>& items, std::vector<double>& res) {
   ent::Engine& E = *ctx->eng;
-  uint64_t total = 0;
-  for (const auto& j : jobs) total = std::max<uint64_t>(total, j.out_off + j.n);
-  DRV_TRY(E.ensure(ctx->out, sizeof(double) * total));
-  DRV_TRY(E.densities(X, jobs, (double*)ctx->out.p, method));
   const uint32_t NI = (uint32_t)items.size();
   uint64_t maxn = 1;
   for (const auto& it : items) maxn = std::max<uint64_t>(maxn, it.n);
@@ -106,6 +101,17 @@ int run_and_reduce(ent_ctx* ctx, const float* X, const std::vector<ent::JobSpec>
   DRV_HIP(hipStreamSynchronize(E.stream()));
   res.assign(h_out, h_out + NI);
   return ENT_OK;
+}
+
+// Densities of all jobs into ctx->out, then the reduction items.
+int run_and_reduce(ent_ctx* ctx, const float* X, const std::vector<ent::JobSpec>& jobs, int method,
+                   const std::vector<ent::RedItem>& items, std::vector<double>& res) {
+  ent::Engine& E = *ctx->eng;
+  uint64_t total = 0;
+  for (const auto& j : jobs) total = std::max<uint64_t>(total, j.out_off + j.n);
+  DRV_TRY(E.ensure(ctx->out, sizeof(double) * total));
+  DRV_TRY(E.densities(X, jobs, (double*)ctx->out.p, method));
+  return reduce_only(ctx, items, res);
 }
 
 ent::RedItem item(uint32_t op, std::initializer_list<uint64_t> offs, uint64_t n, double pref, double scale) {
@@ -391,6 +397,110 @@ int ent_batch_loo_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, siz
   std::vector<double> res;
   DRV_TRY(run_and_reduce(ctx, X_dev, js, method, items, res));
   for (size_t j = 0; j < n_jobs; ++j) loglik[j] = res[j];
+  return ENT_OK;
+}
+
+// AutoEnt::main (autoent.cpp:36-55): per column i, the lagged 1-D density
+// p(x_{t+tau}) and the 2-D density p(x_t, x_{t+tau}), both over the
+// n_rows - tau lagged rows; one batch for all columns.
+int ent_autoent_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
+                    uint32_t tau, int bits, int method, double* out) {
+  DRV_TRY(check_common(ctx, X_dev, n_rows, n_cols, h));
+  if (!out) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  DRV_TRY(check_taus(n_rows, &tau, 1));
+  auto t0 = std::chrono::steady_clock::now();
+  const double sc = bits ? 1.0 / std::log(2.0) : 1.0;
+  const uint64_t m = n_rows - tau;
+  std::vector<ent::JobSpec> jobs;
+  std::vector<ent::RedItem> items;
+  for (uint32_t i = 0; i < n_cols; ++i) {
+    const uint64_t o2 = 2 * (uint64_t)i * m, o1 = o2 + m;
+    jobs.push_back(make_job(n_rows, {i, i}, {h[i], h[i]}, {0, tau}, o2));
+    jobs.push_back(make_job(n_rows, {i}, {h[i]}, {tau}, o1));
+    items.push_back(item(ent::kRedAutoEnt, {o2, o1}, m, (double)h[i], sc));
+  }
+  std::vector<double> res;
+  DRV_TRY(run_and_reduce(ctx, X_dev, jobs, method, items, res));
+  for (size_t i = 0; i < n_cols; ++i) out[i] = res[i];
+  ctx->driver_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return ENT_OK;
+}
+
+namespace {
+
+// 1-D densities of every column into ctx->out (column c at c*n_rows) and
+// the reciprocal of each one's sum (Tools::sum1_normalized, tools.hxx:33-43).
+int column_densities_normalised(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols,
+                                const float* h, int method, std::vector<double>& inv_sum) {
+  std::vector<ent::JobSpec> jobs;
+  std::vector<ent::RedItem> items;
+  for (uint32_t c = 0; c < n_cols; ++c) {
+    jobs.push_back(make_job(n_rows, {c}, {h[c]}, {0}, (uint64_t)c * n_rows));
+    items.push_back(item(ent::kRedSum, {(uint64_t)c * n_rows}, n_rows, 0.0, 1.0));
+  }
+  std::vector<double> sums;
+  DRV_TRY(run_and_reduce(ctx, X_dev, jobs, method, items, sums));
+  inv_sum.resize(n_cols);
+  for (size_t c = 0; c < n_cols; ++c) inv_sum[c] = sums[c] != 0.0 ? 1.0 / sums[c] : 1.0;
+  return ENT_OK;
+}
+
+}  // namespace
+
+// KLDiv::main (kldiv.cpp:34-50): for every ordered column pair,
+// sum_k p_i log(p_i / p_j) over sum1-normalised 1-D densities.  The
+// densities are computed once per column (the reference recomputes p_j for
+// every pair, kldiv.cpp:42-46).
+int ent_kldiv_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h, int bits,
+                  int method, double* out) {
+  DRV_TRY(check_common(ctx, X_dev, n_rows, n_cols, h));
+  if (!out) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  const double sc = bits ? 1.0 / std::log(2.0) : 1.0;
+  std::vector<double> inv;
+  DRV_TRY(column_densities_normalised(ctx, X_dev, n_rows, n_cols, h, method, inv));
+  std::vector<ent::RedItem> items;
+  for (uint64_t i = 0; i < n_cols; ++i)
+    for (uint64_t j = 0; j < n_cols; ++j) {
+      if (i == j) continue;
+      ent::RedItem it = item(ent::kRedKL, {i * n_rows, j * n_rows}, n_rows, inv[i], sc);
+      it.pref2 = inv[j];
+      items.push_back(it);
+    }
+  std::vector<double> res;
+  DRV_TRY(reduce_only(ctx, items, res));
+  size_t p = 0;
+  for (size_t i = 0; i < n_cols; ++i)
+    for (size_t j = 0; j < n_cols; ++j) out[i * n_cols + j] = (i == j) ? 0.0 : res[p++];
+  ctx->driver_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return ENT_OK;
+}
+
+// Negs::main (negs.cpp:34-46): per column, -sum_k p log p over the
+// sum1-normalised 1-D density.
+int ent_negs_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h, int bits,
+                 int method, double* out) {
+  DRV_TRY(check_common(ctx, X_dev, n_rows, n_cols, h));
+  if (!out) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  const double sc = bits ? 1.0 / std::log(2.0) : 1.0;
+  std::vector<double> inv;
+  DRV_TRY(column_densities_normalised(ctx, X_dev, n_rows, n_cols, h, method, inv));
+  std::vector<ent::RedItem> items;
+  for (uint64_t c = 0; c < n_cols; ++c) items.push_back(item(ent::kRedNegs, {c * n_rows}, n_rows, inv[c], sc));
+  std::vector<double> res;
+  DRV_TRY(reduce_only(ctx, items, res));
+  for (size_t c = 0; c < n_cols; ++c) out[c] = -res[c];
+  ctx->driver_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return ENT_OK;
 }
 

@@ -45,6 +45,15 @@ __device__ __forceinline__ double term(const RedItem& it, uint64_t k, const doub
     }
     case kRedSum:
       return a * it.scale;
+    case kRedKL: {  // select_log: log of a zero ratio is 0
+      const double qa = a * it.pref;
+      const double r = qa / (D[it.off[1] + k] * it.pref2);
+      return r == 0.0 ? 0.0 : qa * log(r) * it.scale;
+    }
+    case kRedNegs: {
+      const double q = a * it.pref;
+      return q == 0.0 ? 0.0 : log(q) * q * it.scale;
+    }
     default:
       return 0.0;
   }

@@ -14,6 +14,9 @@ enum RedOp : uint32_t {
                     // leave-one-out log-likelihood (p includes the self pair
                     // K0 = prod 0.75/h); pref = K0, scale = floor density
   kRedSum = 5,      // sum p (epa self-convolution rows, densities_omp.cpp:271-276)
+  kRedKL = 6,       // sum q_a log(q_a / q_b), q_a = p_a * pref, q_b = p_b * pref2
+                    // (sum1-normalised densities, kldiv.cpp:49-51)
+  kRedNegs = 7,     // sum log(q) q, q = p * pref (negs.cpp:43-45)
 };
 
 // off[0..3] index density arrays in one device buffer D (doubles).
@@ -22,8 +25,9 @@ struct RedItem {
   uint64_t n;
   uint32_t op;
   uint32_t pad;
-  double pref;   // TE prefactor / autoent bandwidth
+  double pref;   // TE prefactor / autoent bandwidth / normalisation of off[0]
   double scale;  // 1 for ln, 1/ln 2 for bits
+  double pref2;  // normalisation of off[1] (KL)
 };
 
 void launch_reduce(hipStream_t s, const RedItem* items, uint32_t n_items, const double* D,

@@ -293,6 +293,44 @@ def te_matrix(ctx: Context, x: DeviceArray, h: Sequence[float], taus: Sequence[i
     return out
 
 
+def _hcheck(x: DeviceArray, h: Sequence[float]) -> np.ndarray:
+    hh = _f32(h)
+    if hh.size != x.shape[0]:
+        raise EntropicalError(_lib.ENT_ERR_ARG, "error: number of given bandwidth values does not match the number of selected columns!")
+    return hh
+
+
+def autoent(ctx: Context, x: DeviceArray, h: Sequence[float], tau: int, bits: bool = False,
+            method: str = "cell") -> np.ndarray:
+    """AutoEnt::main (autoent.cpp:16-63) for every column, one batch."""
+    hh = _hcheck(x, h)
+    n_cols, n_rows = x.shape
+    out = np.empty(n_cols, dtype=np.float64)
+    check(load().ent_autoent_dev(ctx.handle, C.c_void_p(x.ptr), n_rows, n_cols, _ptr(hh), int(tau), int(bits),
+                                 _METHODS[method], _ptr(out)))
+    return out
+
+
+def kldiv(ctx: Context, x: DeviceArray, h: Sequence[float], bits: bool = False, method: str = "cell") -> np.ndarray:
+    """KLDiv::main (kldiv.cpp:14-63): (n_cols, n_cols) matrix, one batch."""
+    hh = _hcheck(x, h)
+    n_cols, n_rows = x.shape
+    out = np.empty((n_cols, n_cols), dtype=np.float64)
+    check(load().ent_kldiv_dev(ctx.handle, C.c_void_p(x.ptr), n_rows, n_cols, _ptr(hh), int(bits),
+                               _METHODS[method], _ptr(out)))
+    return out
+
+
+def negs(ctx: Context, x: DeviceArray, h: Sequence[float], bits: bool = False, method: str = "cell") -> np.ndarray:
+    """Negs::main (negs.cpp:14-53): per-column value, one batch."""
+    hh = _hcheck(x, h)
+    n_cols, n_rows = x.shape
+    out = np.empty(n_cols, dtype=np.float64)
+    check(load().ent_negs_dev(ctx.handle, C.c_void_p(x.ptr), n_rows, n_cols, _ptr(hh), int(bits),
+                              _METHODS[method], _ptr(out)))
+    return out
+
+
 def hestimate(ctx: Context, x: DeviceArray, kind: str = "amise") -> np.ndarray:
     """Bandwidth per column (Hestimate::main, hestimate.hxx:6-24): "thumb"
     (hestimate.cpp:19-30) or "amise" (hestimate.cpp:41-104, all columns'

@@ -185,6 +185,25 @@ int ent_batch_entropy_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs,
 int ent_batch_loo_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs, double floor_rel,
                       int method, double* loglik);
 
+/* AutoEnt::main (autoent.cpp:16-63) for every column of device-resident X in
+ * one batch: out[i] = sum_k [p2 > 0] p2 h h log(p1 inv(p2 h)), p1 = 1-D
+ * density of x_{t+tau}, p2 = 2-D density of (x_t, x_{t+tau}), both over the
+ * n_rows - tau lagged rows (FP64 reduction; the reference accumulates float
+ * logs in double).  bits: log2 instead of ln (select_log, tools.cpp:18-37). */
+int ent_autoent_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
+                    uint32_t tau, int bits, int method, double* out);
+
+/* KLDiv::main (kldiv.cpp:14-63): out[i*n_cols + j] = sum_k p_i log(p_i / p_j)
+ * over sum1-normalised 1-D densities (tools.hxx:33-43), diagonal 0. */
+int ent_kldiv_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h, int bits,
+                  int method, double* out);
+
+/* Negs::main (negs.cpp:14-53): out[i] = -sum_k p log p over the
+ * sum1-normalised 1-D density of column i (FP64; the reference accumulates
+ * in float). */
+int ent_negs_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h, int bits,
+                 int method, double* out);
+
 /* Bandwidth per column of device-resident column-major X, as Hestimate::main
  * computes it for every selected column (hestimate.hxx:6-24):
  * ENT_HEST_THUMB: N^(-1/7) * population sd (Thumb::Kernel,

@@ -185,3 +185,58 @@ def test_sort_variants_bit_identical(ctx, orc, monkeypatch, sort_mode, scatter):
         p = orc.density_fp64(x, n, cols, s["h"], taus if s["n_dim"] == 3 else None)
         g = got[s["out_off"]: s["out_off"] + s["n"]]
         assert np.max(np.abs(g - p) / p) < 1e-5, k
+
+
+# ------------------------------------------------- autoent / kldiv / negs
+def _distinct_columns(seed, n):
+    r = np.random.default_rng(seed)
+    x = np.stack([
+        r.standard_normal(n),
+        np.where(r.random(n) < 0.4, -1.5, 1.0) + 0.5 * r.standard_normal(n),
+        0.7 * r.standard_t(5, n),
+        np.cumsum(r.standard_normal(n)) * 0.02 + r.standard_normal(n),  # autocorrelated
+    ])
+    return x.astype(np.float32)
+
+
+@pytest.mark.parametrize("bits", [False, True])
+def test_autoent(ctx, orc, bits):
+    from entropical_amd import DeviceArray, autoent
+
+    n, tau = 5000, 3
+    x = _distinct_columns(21, n)
+    h = [0.3, 0.25, 0.35, 0.3]
+    dx = DeviceArray(ctx, x)
+    try:
+        got = autoent(ctx, dx, h, tau, bits=bits)
+    finally:
+        dx.free()
+    for i in range(4):
+        p1 = orc.density_fp64(x, n, [i], [h[i]], [tau])
+        p2 = orc.density_fp64(x, n, [i, i], [h[i], h[i]], [0, tau])
+        ref = orc.autoent_ref(p1, p2, h[i], bits)
+        assert abs(got[i] - ref) <= RTOL * abs(ref), (i, got[i], ref)
+
+
+@pytest.mark.parametrize("bits", [False, True])
+def test_kldiv_and_negs(ctx, orc, bits):
+    from entropical_amd import DeviceArray, kldiv, negs
+
+    n = 6000
+    x = _distinct_columns(22, n)
+    h = [0.3, 0.25, 0.35, 0.3]
+    dx = DeviceArray(ctx, x)
+    try:
+        kl = kldiv(ctx, dx, h, bits=bits)
+        ng = negs(ctx, dx, h, bits=bits)
+    finally:
+        dx.free()
+    p = [orc.sum1_normalized(orc.density_fp64(x, n, [i], [h[i]])) for i in range(4)]
+    assert np.all(np.diag(kl) == 0)
+    for i in range(4):
+        ref = orc.negs_ref(p[i], bits)
+        assert abs(ng[i] - ref) <= RTOL * abs(ref), (i, ng[i], ref)
+        for j in range(i + 1, 4):
+            ij, ji = orc.kldiv_ref(p[i], p[j], bits)
+            assert abs(kl[i, j] - ij) <= RTOL * abs(ij), (i, j, kl[i, j], ij)
+            assert abs(kl[j, i] - ji) <= RTOL * abs(ji), (j, i, kl[j, i], ji)
