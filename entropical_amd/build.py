@@ -95,6 +95,19 @@ def build_dropin_check(verbose: bool = True) -> Path:
     return exe
 
 
+def build_cli(verbose: bool = True) -> Path:
+    """The entropical command-line program over the library
+    (entropical_amd/bin/entropical)."""
+    src = PKG / "cli" / "entropical_main.cpp"
+    exe = PKG / "bin" / "entropical"
+    exe.parent.mkdir(exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(src), f"-L{PKG}", "-lentropical_kde",
+          "-Wl,-rpath,$ORIGIN/..", "-o", str(exe)])
+    if verbose:
+        print(f"built {exe}")
+    return exe
+
+
 def build_oracle(verbose: bool = True, with_ref: bool = True) -> None:
     od = ROOT / "oracle"
     _run(["make", "-C", str(od), "-j4", "all"])
@@ -111,6 +124,7 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     build_library(a.j)
     build_dropin_check()
+    build_cli()
     if a.oracle:
         build_oracle()
     return 0
