@@ -4,7 +4,7 @@
 // per bandwidth m0, row boxes per bandwidth mr, tile extent along dim 0 in
 // bandwidths group_h.  Every setting of the grid is timed on a subsample of
 // the batch (after one untimed run, so graph capture and workspace growth are
-// excluded); the four fastest are timed again three times each.
+// excluded); the four fastest are timed again on the whole batch.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
@@ -74,21 +74,23 @@ int Engine::autotune(const float* X, const std::vector<JobSpec>& jobs, double* d
     return kOk;
   }
 
-  // time of `timed` runs after one untimed run (graph capture, workspace
-  // growth), best of the timed runs
-  auto eval = [&](const Tuning& t, int timed, double* ms) -> int {
+  // time of `timed` runs of `set` after one untimed run (graph capture,
+  // workspace growth), best of the timed runs
+  auto eval = [&](const Tuning& t, int timed, double* ms, const std::vector<JobSpec>& set) -> int {
     tuning = t;
     double best = 1e300;
     for (int rep = 0; rep <= timed; ++rep) {
-      int rc = densities(X, sample, dens, kMethodCell);
+      int rc = densities(X, set, dens, kMethodCell);
       if (rc != kOk) return rc;
       if (rep > 0) best = std::min(best, last.total_ms);
     }
     *ms = best;
     return kOk;
   };
-  // Full grid (the axes interact: q bounds a tile's size, m0 * group_h its
-  // extent along dim 0, mr the row height), then the four fastest re-timed.
+  // Full grid on the subsample (the axes interact: q bounds a tile's size,
+  // m0 * group_h its extent along dim 0, mr the row height), then the four
+  // fastest re-timed on the whole batch: differences of a few percent are
+  // below the subsample's run-to-run noise.
   const std::vector<int> qs = {1, 2, 4}, m0s = {4, 8, 16}, mrs = {1, 2}, ghs = {2, 4, 8};
   std::vector<std::pair<double, Tuning>> scored;
   for (int q : qs)
@@ -101,7 +103,7 @@ int Engine::autotune(const float* X, const std::vector<JobSpec>& jobs, double* d
           t.mr = mr;
           t.group_h = gh;
           double ms = 0.0;
-          int rc = eval(t, 1, &ms);
+          int rc = eval(t, 1, &ms, sample);
           if (rc != kOk) return rc;
           scored.emplace_back(ms, t);
         }
@@ -111,7 +113,7 @@ int Engine::autotune(const float* X, const std::vector<JobSpec>& jobs, double* d
   double cur_ms = 1e300;
   for (size_t k = 0; k < std::min<size_t>(4, scored.size()); ++k) {
     double ms = 0.0;
-    int rc = eval(scored[k].second, 3, &ms);
+    int rc = eval(scored[k].second, 2, &ms, jobs);
     if (rc != kOk) return rc;
     if (ms < cur_ms) {
       cur_ms = ms;
