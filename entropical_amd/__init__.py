@@ -13,6 +13,7 @@ from .kde import (  # noqa: F401
     hestimate,
     kldiv,
     negs,
+    read_xtc,
     mi_autotune,
     te_autotune,
     last_driver_ms,
@@ -48,6 +49,7 @@ __all__ = [
     "hestimate",
     "kldiv",
     "negs",
+    "read_xtc",
     "mi_autotune",
     "te_autotune",
     "batch_autotune",

@@ -90,6 +90,7 @@ SIGNATURES = {
     "ent_ctx_last_driver_ms": (C.c_int, [P, F64P]),
     "ent_batch_loo_dev": (C.c_int, [P, P, C.POINTER(ent_job), C.c_size_t, C.c_double, C.c_int, P]),
     "ent_hestimate_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, C.c_int, P]),
+    "ent_read_xtc": (C.c_int, [C.c_char_p, P, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_uint32)]),
     "ent_autoent_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, C.c_uint32, C.c_int, C.c_int, P]),
     "ent_kldiv_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, C.c_int, C.c_int, P]),
     "ent_negs_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, C.c_int, C.c_int, P]),

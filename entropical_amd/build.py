@@ -24,7 +24,7 @@ ARCH = os.environ.get("ENT_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["kde_kernels.hip", "engine.hip", "autotune.hip", "capi.hip", "reductions.hip", "drivers.hip",
                "mlp.hip"]
-HOST_SOURCES = ["densities_shim.cpp"]
+HOST_SOURCES = ["densities_shim.cpp", "xtc.cpp"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

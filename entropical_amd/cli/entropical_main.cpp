@@ -6,7 +6,8 @@
 // densities in one engine batch on the GPU and reduces them there.
 //
 //   modes:   transs autoent kldiv mi dens negs amise thumb
-//   options: -i/--input FILE   coordinates, column-oriented whitespace-separated text
+//   options: -i/--input FILE   coordinates, column-oriented whitespace-separated text,
+//                              or a GROMACS .xtc trajectory (row = frame, column 3a+e)
 //            -o/--output FILE  output file (default: stdout)
 //            -c/--columns "i j ..."  1-based column selection (default: all)
 //            -H/--bandwidths FILE   bandwidth per selected column, whitespace-separated
@@ -208,6 +209,10 @@ struct Coords {
   std::vector<size_t> selected;  // 1-based, as given
 };
 
+bool is_xtc(const std::string& fname) {
+  return fname.size() > 4 && fname.compare(fname.size() - 4, 4, ".xtc") == 0;
+}
+
 Coords read_coords(const std::string& fname, const std::string& columns) {
   Coords C;
   for (const std::string& c : split_ws(columns)) {
@@ -218,30 +223,49 @@ Coords read_coords(const std::string& fname, const std::string& columns) {
     if (v <= 0) die("error: specify column in range of [1, N_COLUMNS]!");
     C.selected.push_back((size_t)v);
   }
-  std::ifstream ifs(fname);
-  if (!ifs) die("error: cannot open input file '" + fname + "'");
-  std::vector<std::vector<float>> rows;
-  std::string line;
-  size_t n_file_cols = 0;
-  while (std::getline(ifs, line)) {
-    if (line.empty()) continue;
-    std::vector<float> r;
-    const char* p = line.c_str();
-    char* end = nullptr;
-    for (;;) {
-      while (*p == ' ' || *p == '\t' || *p == '\r') ++p;
-      if (!*p) break;
-      float f = std::strtof(p, &end);
-      if (end == p) die("error: non-numeric value in input file '" + fname + "'");
-      r.push_back(f);
-      p = end;
+  // the whole file, column-major: full[c * n_rows + r]
+  std::vector<float> full;
+  size_t n_rows = 0, n_file_cols = 0;
+  if (is_xtc(fname)) {
+    // GROMACS trajectory: frame = row, coordinate e of atom a = column 3a+e
+    // (CoordsFile::XtcHandler, coords_file/coords_file.cpp:95-114)
+    size_t nf = 0;
+    uint32_t na = 0;
+    if (ent_read_xtc(fname.c_str(), nullptr, 0, &nf, &na) != ENT_OK) die(std::string("error: ") + ent_last_error());
+    full.resize(nf * 3 * (size_t)na);
+    if (ent_read_xtc(fname.c_str(), full.data(), full.size(), &nf, &na) != ENT_OK)
+      die(std::string("error: ") + ent_last_error());
+    n_rows = nf;
+    n_file_cols = 3 * (size_t)na;
+  } else {
+    std::ifstream ifs(fname);
+    if (!ifs) die("error: cannot open input file '" + fname + "'");
+    std::vector<std::vector<float>> rows;
+    std::string line;
+    while (std::getline(ifs, line)) {
+      if (line.empty()) continue;
+      std::vector<float> r;
+      const char* p = line.c_str();
+      char* end = nullptr;
+      for (;;) {
+        while (*p == ' ' || *p == '\t' || *p == '\r') ++p;
+        if (!*p) break;
+        float f = std::strtof(p, &end);
+        if (end == p) die("error: non-numeric value in input file '" + fname + "'");
+        r.push_back(f);
+        p = end;
+      }
+      if (r.empty()) continue;
+      if (rows.empty()) n_file_cols = r.size();
+      if (r.size() != n_file_cols)
+        die("error: row " + std::to_string(rows.size() + 1) + " of '" + fname + "' has " + std::to_string(r.size()) +
+            " values, expected " + std::to_string(n_file_cols));
+      rows.push_back(std::move(r));
     }
-    if (r.empty()) continue;
-    if (rows.empty()) n_file_cols = r.size();
-    if (r.size() != n_file_cols)
-      die("error: row " + std::to_string(rows.size() + 1) + " of '" + fname + "' has " + std::to_string(r.size()) +
-          " values, expected " + std::to_string(n_file_cols));
-    rows.push_back(std::move(r));
+    n_rows = rows.size();
+    full.resize(n_rows * n_file_cols);
+    for (size_t r = 0; r < n_rows; ++r)
+      for (size_t c = 0; c < n_file_cols; ++c) full[c * n_rows + r] = rows[r][c];
   }
   std::vector<size_t> use;  // 0-based file columns, file order
   if (C.selected.empty()) {
@@ -259,11 +283,11 @@ Coords read_coords(const std::string& fname, const std::string& columns) {
     }
     use.assign(s.begin(), s.end());
   }
-  C.n_rows = rows.size();
+  C.n_rows = n_rows;
   C.n_cols = use.size();
   C.x.resize(C.n_rows * C.n_cols);
-  for (size_t r = 0; r < C.n_rows; ++r)
-    for (size_t k = 0; k < C.n_cols; ++k) C.x[k * C.n_rows + r] = rows[r][use[k]];
+  for (size_t k = 0; k < C.n_cols; ++k)
+    std::memcpy(C.x.data() + k * n_rows, full.data() + use[k] * n_rows, sizeof(float) * n_rows);
   return C;
 }
 

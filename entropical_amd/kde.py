@@ -293,6 +293,17 @@ def te_matrix(ctx: Context, x: DeviceArray, h: Sequence[float], taus: Sequence[i
     return out
 
 
+def read_xtc(path) -> np.ndarray:
+    """XTC trajectory as a column-major (3*natoms, n_frames) float32 array:
+    column 3a + e holds coordinate e of atom a (coords_file.cpp:95-114)."""
+    nf, na = C.c_size_t(), C.c_uint32()
+    p = str(path).encode()
+    check(load().ent_read_xtc(p, None, 0, C.byref(nf), C.byref(na)))
+    out = np.empty((3 * na.value, nf.value), dtype=np.float32)
+    check(load().ent_read_xtc(p, _ptr(out), out.size, C.byref(nf), C.byref(na)))
+    return out
+
+
 def _hcheck(x: DeviceArray, h: Sequence[float]) -> np.ndarray:
     hh = _f32(h)
     if hh.size != x.shape[0]:

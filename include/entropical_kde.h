@@ -204,6 +204,15 @@ int ent_kldiv_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols
 int ent_negs_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h, int bits,
                  int method, double* out);
 
+/* GROMACS XTC trajectory (CoordsFile::XtcHandler, coords_file/coords_file.cpp:72-127,
+ * over the vendored xdrfile library) read on the host into the column-major
+ * coordinate layout: frame f is row f, coordinate e of atom a is column
+ * 3a + e (the row layout of XtcHandler::next).  Call with out == NULL for
+ * *n_frames and *natoms, then with capacity >= n_frames * 3 * natoms floats.
+ * Own decoder of the format (no xdrfile); all frames must have the same
+ * atom count. */
+int ent_read_xtc(const char* path, float* out, size_t capacity, size_t* n_frames, uint32_t* natoms);
+
 /* Bandwidth per column of device-resident column-major X, as Hestimate::main
  * computes it for every selected column (hestimate.hxx:6-24):
  * ENT_HEST_THUMB: N^(-1/7) * population sd (Thumb::Kernel,

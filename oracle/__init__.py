@@ -268,3 +268,45 @@ def ref_boxes(coords, n_rows, col, box_size):
     box = np.empty(n_rows, dtype=np.int64)
     nb = ref().ref_boxes(_p(flat), n_rows, col, float(box_size), _p(box))
     return box, int(nb)
+
+
+# ------------------------------------------------ reference XTC reader/writer
+# The reference's vendored xdrfile library (coords_file/xdrfile), compiled
+# into _ref/libref_xdr.so by `make -C oracle ref` where the reference tree is
+# present.
+REF_XDR = HERE / "_ref" / "libref_xdr.so"
+_xdr = None
+
+
+def xdr_available() -> bool:
+    return REF_XDR.exists()
+
+
+def _xdr_lib() -> C.CDLL:
+    global _xdr
+    if _xdr is None:
+        L = C.CDLL(str(REF_XDR))
+        L.refx_write.restype = C.c_int
+        L.refx_write.argtypes = [C.c_char_p, C.c_int, C.c_int, P, C.c_float]
+        L.refx_read.restype = C.c_int
+        L.refx_read.argtypes = [C.c_char_p, C.c_int, P, C.POINTER(C.c_int)]
+        _xdr = L
+    return _xdr
+
+
+def ref_write_xtc(path, xyz, precision=1000.0) -> None:
+    """xyz: (n_frames, natoms, 3) float32, written with the reference writer."""
+    a = np.ascontiguousarray(xyz, dtype=np.float32)
+    rc = _xdr_lib().refx_write(str(path).encode(), a.shape[1], a.shape[0], _p(a), float(precision))
+    if rc != 0:
+        raise RuntimeError(f"reference XTC writer failed ({rc})")
+
+
+def ref_read_xtc(path, max_frames: int, natoms: int) -> np.ndarray:
+    """(n_frames, natoms, 3) float32 read with the reference reader."""
+    out = np.empty((max_frames, natoms, 3), dtype=np.float32)
+    nf = C.c_int()
+    na = _xdr_lib().refx_read(str(path).encode(), max_frames, _p(out), C.byref(nf))
+    if na != natoms:
+        raise RuntimeError(f"reference XTC reader returned {na} atoms")
+    return out[: nf.value]

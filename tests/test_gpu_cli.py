@@ -119,3 +119,23 @@ def test_thumb_and_amise(data, orc):
         assert th[i] == pytest.approx(orc.thumb(x[i]), rel=2e-6)
         h0 = orc.thumb(x[i])
         assert abs(am[i] - orc.amise(x[i], exact=True)) <= 0.05 * h0
+
+
+def test_xtc_input(orc, tmp_path):
+    """.xtc input (the committed fixture written by the reference's XTC
+    writer): columns are atom coordinates 3a+e of each frame."""
+    gold = Path(__file__).resolve().parent / "golden"
+    exp = np.load(gold / "xtc_walk_400x10.expected.npy")  # (frames, atoms, 3)
+    cols = exp.reshape(exp.shape[0], -1).T.astype(np.float32)  # (30, frames)
+    n = cols.shape[1]
+    sel = [1, 2, 6, 14]  # 1-based columns
+    h = [0.3, 0.3, 0.4, 0.35]
+    hfile = tmp_path / "h.txt"
+    hfile.write_text(" ".join(f"{v}" for v in h))
+    m = matrix(cli("mi", "-i", gold / "xtc_walk_400x10.xtc", "-c", " ".join(map(str, sel)), "-H", hfile))
+    x = cols[[c - 1 for c in sel]]
+    p = [orc.density_fp64(x, n, [i], [h[i]]) for i in range(4)]
+    for i in range(4):
+        for j in range(i + 1, 4):
+            ref = orc.mi_fp64(orc.density_fp64(x, n, [i, j], [h[i], h[j]]), p[i], p[j], False)
+            assert m[i, j] == pytest.approx(ref, rel=RTOL)
