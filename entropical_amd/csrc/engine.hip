@@ -351,7 +351,7 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   ENT_TRY(ensure(d_scoord, sizeof(float) * 3 * P));
   ENT_TRY(ensure(d_tile_cnt, sizeof(uint32_t) * (NC + 1)));
   ENT_TRY(ensure(d_tiles, sizeof(DTile) * P));
-  ENT_TRY(ensure(d_counter, 64));
+  ENT_TRY(ensure(d_counter, sizeof(uint32_t) * kCounterWords));
   ENT_TRY(ensure(d_cellstart, sizeof(uint32_t) * std::max<uint64_t>(table_total, 1)));
   if (shared) {
     for (int b = 0; b < 2; ++b) {
@@ -505,7 +505,7 @@ int Engine::run_brute(const float* X, const std::vector<JobSpec>& jobs, double* 
   ENT_TRY(ensure(d_jobs, sizeof(DJob) * J));
   ENT_TRY(ensure(d_tiles, sizeof(DTile) * T));
   ENT_TRY(ensure(d_tile_cnt, sizeof(uint32_t) * 2));
-  ENT_TRY(ensure(d_counter, 64));
+  ENT_TRY(ensure(d_counter, sizeof(uint32_t) * kCounterWords));
   size_t stage_bytes = sizeof(DJob) * J + sizeof(DTile) * T + 16;
   ENT_TRY(ensure_host(stage_bytes));
   char* hs = (char*)host_stage_;
@@ -534,10 +534,10 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
   const uint32_t* n_tiles = raw ? (const uint32_t*)d_tile_cnt.p : (const uint32_t*)d_tile_cnt.p + NC;
   uint32_t* counter = (uint32_t*)d_counter.p;
   int rc = 0;
-  // the work counter and pair counters are cleared by a fill kernel (not a
+  // the claim counters and pair counters are cleared by a fill kernel (not a
   // memset node) inside the same graph as the persistent launch
   auto issue = [&] {
-    launch_fill_u32(stream_, counter, 0u, 16);
+    launch_fill_u32(stream_, counter, 0u, kCounterWords);
     rc = launch_density(stream_, tuning.q, raw, wide, grid, (const DJob*)d_jobs.p, (const DTile*)d_tiles.p,
                         n_tiles, counter, X, raw ? nullptr : (const float*)d_scoord.p,
                         raw ? nullptr : d_keys[sorted_buf].p, raw ? nullptr : (const uint32_t*)d_cellstart.p,

@@ -13,10 +13,6 @@
 #ifndef ENT_DENSITY_MIN_WAVES
 #define ENT_DENSITY_MIN_WAVES 1
 #endif
-// LDS candidate reads issued ahead of their first use in the inner loop.
-#ifndef ENT_LDS_AHEAD
-#define ENT_LDS_AHEAD 1
-#endif
 
 namespace ent {
 namespace {
@@ -676,67 +672,63 @@ __device__ __forceinline__ void accumulate_chunk(const WaveLds& L, const float (
   for (int q = 0; q < Q; ++q) part[q] = 0.0f;
   constexpr int kPer4 = (D == 1) ? 4 : (D == 2 ? 2 : 1);
   constexpr int kReads = kWave / kPer4;  // float4 LDS reads per chunk
-  constexpr int kAhead = D == 3 ? 1 : ENT_LDS_AHEAD;  // reads issued before the first use
+
+  // the candidates of one float4 read against the lane's queries
+  auto consume = [&](const float4 v, const int g) {
+    float cs[4][3];
+    if constexpr (D == 1) {
+      cs[0][0] = v.x; cs[1][0] = v.y; cs[2][0] = v.z; cs[3][0] = v.w;
+    } else if constexpr (D == 2) {
+      cs[0][0] = v.x; cs[0][1] = v.y; cs[1][0] = v.z; cs[1][1] = v.w;
+    } else {
+      cs[0][0] = v.x; cs[0][1] = v.y; cs[0][2] = v.z;
+    }
+    if constexpr (D == 1) {
+      // four candidates per read: the terms are summed as a small tree so
+      // the accumulator sees one dependent add per four candidates (a
+      // serial chain sub -> fma -> add per candidate leaves the SIMD idle
+      // between dependent instructions)
 #pragma unroll
-  for (int g = 0; g < kReads; g += kAhead) {
-    float4 vv[kAhead];
+      for (int q = 0; q < Q; ++q) {
+        float w4[4];
 #pragma unroll
-    for (int u = 0; u < kAhead; ++u) vv[u] = c4[g + u];
-#pragma unroll
-    for (int u = 0; u < kAhead; ++u) {
-      const float4 v = vv[u];
-      float cs[4][3];
-      if constexpr (D == 1) {
-        cs[0][0] = v.x; cs[1][0] = v.y; cs[2][0] = v.z; cs[3][0] = v.w;
-      } else if constexpr (D == 2) {
-        cs[0][0] = v.x; cs[0][1] = v.y; cs[1][0] = v.z; cs[1][1] = v.w;
-      } else {
-        cs[0][0] = v.x; cs[0][1] = v.y; cs[0][2] = v.z;
+        for (int c = 0; c < 4; ++c) {
+          float uu = sq[0][q] - cs[c][0];
+          if constexpr (RAW) uu = uu * hinv[0];
+          if constexpr (TRI) w4[c] = clamp01(1.0f - fabsf(uu));
+          else w4[c] = clamp01(__builtin_fmaf(-uu, uu, 1.0f));
+        }
+        part[q] += (w4[0] + w4[1]) + (w4[2] + w4[3]);
       }
-      if constexpr (D == 1) {
-        // four candidates per read: the terms are summed as a small tree so
-        // the accumulator sees one dependent add per four candidates (a
-        // serial chain sub -> fma -> add per candidate leaves the SIMD idle
-        // between dependent instructions)
+    } else {
+#pragma unroll
+      for (int c = 0; c < kPer4; ++c) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-          float w4[4];
+          float w[3];
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            float uu = sq[0][q] - cs[c][0];
-            if constexpr (RAW) uu = uu * hinv[0];
-            if constexpr (TRI) w4[c] = clamp01(1.0f - fabsf(uu));
-            else w4[c] = clamp01(__builtin_fmaf(-uu, uu, 1.0f));
+          for (int e = 0; e < D; ++e) {
+            float uu = sq[e][q] - cs[c][e];
+            if constexpr (RAW) uu = uu * hinv[e];
+            w[e] = clamp01(__builtin_fmaf(-uu, uu, 1.0f));
           }
-          part[q] += (w4[0] + w4[1]) + (w4[2] + w4[3]);
-        }
-      } else {
-#pragma unroll
-        for (int c = 0; c < kPer4; ++c) {
-#pragma unroll
-          for (int q = 0; q < Q; ++q) {
-            float w[3];
-#pragma unroll
-            for (int e = 0; e < D; ++e) {
-              float uu = sq[e][q] - cs[c][e];
-              if constexpr (RAW) uu = uu * hinv[e];
-              w[e] = clamp01(__builtin_fmaf(-uu, uu, 1.0f));
-            }
-            if constexpr (D == 2) part[q] = __builtin_fmaf(w[0], w[1], part[q]);
-            else part[q] = __builtin_fmaf(w[0] * w[1], w[2], part[q]);
-          }
-        }
-      }
-      // compensated merge every 32 candidates
-      if (((g + u + 1) * kPer4) % 32 == 0) {
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-          kahan_add(S[q], C[q], part[q]);
-          part[q] = 0.0f;
+          if constexpr (D == 2) part[q] = __builtin_fmaf(w[0], w[1], part[q]);
+          else part[q] = __builtin_fmaf(w[0] * w[1], w[2], part[q]);
         }
       }
     }
-  }
+    // compensated merge every 32 candidates
+    if (((g + 1) * kPer4) % 32 == 0) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        kahan_add(S[q], C[q], part[q]);
+        part[q] = 0.0f;
+      }
+    }
+  };
+
+#pragma unroll
+  for (int g = 0; g < kReads; ++g) consume(c4[g], g);
 }
 
 // Densities of one tile's queries; returns the tile's candidate count
@@ -868,23 +860,26 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
     }
   };
 
-  float nxt[3];
-  load_cand((uint32_t)l, nxt);
-  for (uint32_t base = 0; base < total; base += kWave) {
+  // one staged chunk: convert the loaded candidates, stage them, reload the
+  // buffer `ahead` chunks further on, run the pair loop
+  auto chunk = [&](uint32_t base, float* buf, uint32_t ahead) {
     float s_c[3];
 #pragma unroll
-    for (int e = 0; e < D; ++e) s_c[e] = RAW ? nxt[e] : (nxt[e] - o[e]) * hinv[e];
+    for (int e = 0; e < D; ++e) s_c[e] = RAW ? buf[e] : (buf[e] - o[e]) * hinv[e];
     wave_sync();
     stage_store<D>(L, l, s_c);
     wave_sync();
-    if (base + kWave < total) load_cand(base + kWave + (uint32_t)l, nxt);
+    if (base + ahead < total) load_cand(base + ahead + (uint32_t)l, buf);
     if constexpr (D == 1) {
       if (tri) accumulate_chunk<1, Q, RAW, true>(L, sq, hinv, S, C);
       else accumulate_chunk<1, Q, RAW, false>(L, sq, hinv, S, C);
     } else {
       accumulate_chunk<D, Q, RAW>(L, sq, hinv, S, C);
     }
-  }
+  };
+  float nxt[3];
+  load_cand((uint32_t)l, nxt);
+  for (uint32_t base = 0; base < total; base += kWave) chunk(base, nxt, kWave);
   wave_sync();
 
 #pragma unroll
@@ -923,7 +918,7 @@ __device__ __forceinline__ void run_tile(const DJob& J, const DTile& T, const fl
 
 }  // namespace
 
-// Persistent density kernel: every wave pulls tiles from a global counter
+// Persistent density kernel: every wave pulls tiles from the claim counters
 // until the tile list is exhausted.
 template <int Q, bool RAW, typename KeyT>
 __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const DJob* __restrict__ jobs,
@@ -943,17 +938,30 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
   const int l = lane_id();
   // executed pairs per dimension and lane slots, flushed once per wave
   uint64_t acc[3] = {0, 0, 0}, slots = 0;
-  for (;;) {
-    uint32_t t = 0;
-    if (l == 0) t = atomicAdd(counter, 1u);
-    t = __builtin_amdgcn_readfirstlane(t);
-    if (t >= n_tiles) break;
-    const DTile T = tiles[t];
-    const DJob J = jobs[T.job];
-    const uint32_t d = __builtin_amdgcn_readfirstlane(J.d);
-    if (d == 1) run_tile<1, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[0], slots);
-    else if (d == 2) run_tile<2, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[1], slots);
-    else run_tile<3, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[2], slots);
+  // Tiles are claimed from kClaimParts contiguous partitions of the tile
+  // list, each with its own counter: a wave starts on the partition of its
+  // workgroup's XCD (workgroups are dealt to the XCDs round robin) and moves
+  // on to the others once that one is exhausted.  One shared counter
+  // serialises the claims of short tiles at the memory's atomic unit.
+  const uint32_t home = blockIdx.x % kClaimParts;
+  for (uint32_t k = 0; k < kClaimParts; ++k) {
+    const uint32_t part = (home + k) % kClaimParts;
+    const uint32_t lo = (uint32_t)((uint64_t)n_tiles * part / kClaimParts);
+    const uint32_t hi = (uint32_t)((uint64_t)n_tiles * (part + 1) / kClaimParts);
+    uint32_t* ctr = counter + part * kClaimStride;
+    for (;;) {
+      uint32_t i = 0;
+      if (l == 0) i = atomicAdd(ctr, 1u);
+      i = __builtin_amdgcn_readfirstlane(i);
+      if (i >= hi - lo) break;
+      const uint32_t t = lo + i;
+      const DTile T = tiles[t];
+      const DJob J = jobs[T.job];
+      const uint32_t d = __builtin_amdgcn_readfirstlane(J.d);
+      if (d == 1) run_tile<1, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[0], slots);
+      else if (d == 2) run_tile<2, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[1], slots);
+      else run_tile<3, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[2], slots);
+    }
   }
   if (l == 0) {
     for (int e = 0; e < 3; ++e)

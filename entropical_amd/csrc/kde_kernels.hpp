@@ -40,6 +40,18 @@ void launch_tile_write(hipStream_t s, bool wide, const DJob* jobs, const uint2* 
                        const void* keys, uint32_t tq, const uint32_t* offs, DTile* tiles);
 void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
                       uint64_t max_tiles);
+// Claim counters of the persistent density kernel: kClaimParts counters
+// kClaimStride words apart (separate memory channels), one per partition of
+// the tile list; words 2..9 hold the pair statistics.  The buffer has
+// kCounterWords words, all cleared before each launch.
+#ifndef ENT_CLAIM_PARTS
+#define ENT_CLAIM_PARTS 8
+#endif
+constexpr uint32_t kClaimPartsMax = 8;
+constexpr uint32_t kClaimParts = ENT_CLAIM_PARTS;
+constexpr uint32_t kClaimStride = 1024;
+constexpr uint32_t kCounterWords = kClaimPartsMax * kClaimStride;
+static_assert(kClaimParts >= 1 && kClaimParts <= kClaimPartsMax, "claim partitions");
 int density_occupancy(int q, bool raw, bool wide, int* blocks_per_cu);
 int launch_density(hipStream_t s, int q, bool raw, bool wide, unsigned grid, const DJob* jobs,
                    const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
