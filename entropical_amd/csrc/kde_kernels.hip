@@ -515,14 +515,39 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
   const int l = lane_id();
   uint32_t* st = start + J.cbase;
   const KeyT* kk = keys + J.pbase;
+  // Point p owns the cells (key[p-1], key[p]]; cells after the last key hold
+  // n.  Short runs are written by their own lane; runs longer than
+  // kLaneRun (empty stretches of a sparse grid) by the whole wave, one run
+  // at a time, so one long run does not serialise on a single lane.
+  constexpr uint64_t kLaneRun = 16;
   for (int i = 0; i < kChunkItems; ++i) {
-    uint32_t p = ch.y + i * kWave + l;
-    if (p >= J.n) break;
-    const uint64_t cur = kk[p];
-    const uint64_t first = p == 0 ? 0 : (uint64_t)kk[p - 1] + 1;
-    for (uint64_t cell = first; cell <= cur; ++cell) st[cell] = p;
-    if (p + 1 == J.n)
-      for (uint64_t cell = cur + 1; cell <= J.cells; ++cell) st[cell] = J.n;
+    const uint32_t p0 = ch.y + (uint32_t)i * kWave;
+    if (p0 >= J.n) break;
+    const uint32_t p = p0 + (uint32_t)l;
+    uint64_t first = 0, cnt = 0;
+    if (p < J.n) {
+      const uint64_t cur = kk[p];
+      first = p == 0 ? 0 : (uint64_t)kk[p - 1] + 1;
+      cnt = cur + 1 - first;
+    }
+    const bool wide_run = cnt > kLaneRun;
+    if (!wide_run)
+      for (uint64_t e = 0; e < cnt; ++e) st[first + e] = p;
+    uint64_t bm = __ballot(wide_run);
+    while (bm) {
+      const int src = __ffsll((unsigned long long)bm) - 1;
+      bm &= bm - 1;
+      const uint64_t f = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(first >> 32), src) << 32) |
+                         __builtin_amdgcn_readlane((uint32_t)first, src);
+      const uint64_t n = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(cnt >> 32), src) << 32) |
+                         __builtin_amdgcn_readlane((uint32_t)cnt, src);
+      const uint32_t v = p0 + (uint32_t)src;
+      for (uint64_t e = (uint64_t)l; e < n; e += kWave) st[f + e] = v;
+    }
+    if (J.n - 1 - p0 < (uint32_t)kWave) {  // this step holds the job's last point
+      const uint64_t tail = (uint64_t)kk[J.n - 1] + 1;
+      for (uint64_t cell = tail + (uint64_t)l; cell <= J.cells; cell += kWave) st[cell] = J.n;
+    }
   }
 }
 
@@ -535,10 +560,12 @@ template <typename KeyT>
 __device__ __forceinline__ bool tile_start(const DJob& J, const KeyT* keys, uint32_t p, uint32_t tq) {
   if (p == 0 || (p % tq) == 0) return true;
   if (J.flags & kJobBrute) return false;
-  uint64_t a = keys[J.pbase + p - 1], b = keys[J.pbase + p];
-  uint64_t ra = a / J.nb[0], rb = b / J.nb[0];
+  // key width arithmetic (32-bit division for 32-bit keys)
+  const KeyT a = keys[J.pbase + p - 1], b = keys[J.pbase + p];
+  const KeyT nb0 = (KeyT)J.nb[0];
+  const KeyT ra = a / nb0, rb = b / nb0;
   if (ra != rb) return true;
-  return ((a - ra * J.nb[0]) / J.group) != ((b - rb * J.nb[0]) / J.group);
+  return (uint32_t)(a - ra * nb0) / J.group != (uint32_t)(b - rb * nb0) / J.group;
 }
 }  // namespace
 
