@@ -407,10 +407,12 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     }
     if (!with_tiles) return;
     if (table_total) launch_cell_starts(stream_, wide, dj, dch, NC, keys[cur], (uint32_t*)d_cellstart.p);
-    launch_tile_count(stream_, wide, dj, dch, NC, keys[cur], tq, tcnt);
+    const uint32_t* cst = (const uint32_t*)d_cellstart.p;
+    launch_tile_count(stream_, wide, dj, dch, NC, keys[cur], cst, tq, tcnt);
     launch_exclusive_scan(stream_, tcnt, tcnt, NC, scratch);
-    launch_tile_write(stream_, wide, dj, dch, NC, keys[cur], tq, tcnt, (DTile*)d_tiles.p);
+    launch_tile_write(stream_, wide, dj, dch, NC, keys[cur], cst, tq, tcnt, (DTile*)d_tiles.p);
     launch_tile_ends(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
+    launch_tile_pack(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
   };
   if (stop == kStopAfterSort) {
     issue_pre(false);
@@ -498,7 +500,7 @@ int Engine::run_brute(const float* X, const std::vector<JobSpec>& jobs, double* 
       t.job = j;
       t.p0 = p;
       t.p1 = std::min(jobs[j].n, p + tq);
-      t.pad = 0;
+      t.pack = 0;
       tiles.push_back(t);
     }
   const uint64_t T = tiles.size();

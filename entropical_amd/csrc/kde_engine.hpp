@@ -57,12 +57,18 @@ constexpr uint64_t kNoTable = ~0ull;
 constexpr uint64_t kTableCellsPerPoint = 8;
 
 // Tile: queries [p0, p1) of job `job` (sorted positions, job-local).
+// `pack` (cell-list tiles only): bits 0-1 hold G - 1 for a packed tile of G
+// = 2 or 4 consecutive sub-tiles of one job processed side by side by one
+// wave (64 / G lanes each); bits 8 s .. 8 s + 7 hold the offset from p0 of
+// sub-tile s >= 1.  kTileSkip marks a tile absorbed into the packed tile
+// before it.
 struct DTile {
   uint32_t job;
   uint32_t p0;
   uint32_t p1;
-  uint32_t pad;
+  uint32_t pack;
 };
+constexpr uint32_t kTileSkip = 2u;  // bits 0-1 = 2: no G of its own
 
 struct Tuning {
   int q = 2;            // maximum queries per lane (tile = 64*q queries)

@@ -13,6 +13,16 @@
 #ifndef ENT_DENSITY_MIN_WAVES
 #define ENT_DENSITY_MIN_WAVES 1
 #endif
+// Packing of small tiles into one wave (0: every tile on its own).
+#ifndef ENT_TILE_PACK
+#define ENT_TILE_PACK 1
+#endif
+// Tile boundaries inside a row are dropped between dim-0 groups whose common
+// aligned block of at most 2^ENT_MERGE_LEVELS groups holds at most one tile
+// of points (0: every group starts a tile).
+#ifndef ENT_MERGE_LEVELS
+#define ENT_MERGE_LEVELS 0
+#endif
 
 namespace ent {
 namespace {
@@ -31,26 +41,98 @@ __device__ __forceinline__ uint32_t f2o(float f) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-__device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, kWave));
-  return v;
+// Cross-lane steps on the DPP network (no LDS round trip).  Controls of the
+// gfx9 encoding: row_shr:n = 0x110 + n, row_ror:n = 0x120 + n,
+// row_bcast:15 = 0x142, row_bcast:31 = 0x143.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROW_MASK, 0xF, false);
 }
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dppf(float old, float v) {
+  return __uint_as_float(dpp<CTRL, ROW_MASK>(__float_as_uint(old), __float_as_uint(v)));
+}
+
+// Wave minimum / maximum, returned wave-uniform: rotations within each row
+// of 16 lanes (every lane has a source), then rows combined through lanes
+// 15 and 31; lane 63 ends with the result.
+template <bool MAX>
+__device__ __forceinline__ float wave_extreme(float v) {
+  auto op = [](float a, float b) { return MAX ? fmaxf(a, b) : fminf(a, b); };
+  v = op(v, dppf<0x121>(v, v));
+  v = op(v, dppf<0x122>(v, v));
+  v = op(v, dppf<0x124>(v, v));
+  v = op(v, dppf<0x128>(v, v));
+  v = op(v, dppf<0x142, 0xA>(v, v));
+  v = op(v, dppf<0x143, 0xC>(v, v));
+  return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), kWave - 1));
+}
+__device__ __forceinline__ float wave_min(float v) { return wave_extreme<false>(v); }
+__device__ __forceinline__ float wave_max(float v) { return wave_extreme<true>(v); }
+
+// Inclusive wave scan of uint32: shifted adds within rows of 16 (lanes
+// shifted in from outside the row add 0), then row totals carried through
+// lanes 15 and 31.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  v += dpp<0x111>(0u, v);
+  v += dpp<0x112>(0u, v);
+  v += dpp<0x114>(0u, v);
+  v += dpp<0x118>(0u, v);
+  v += dpp<0x142, 0xA>(0u, v);
+  v += dpp<0x143, 0xC>(0u, v);
   return v;
 }
 
-// Inclusive wave scan of uint32.
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  const int l = lane_id();
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    uint32_t t = __shfl_up(v, o, kWave);
-    if (l >= o) v += t;
-  }
+// Reductions and scans over groups of W = 16 or 32 consecutive lanes (the
+// sub-tiles of a packed tile); every lane of a group gets its group's
+// result.  Rotations within rows of 16, then for W = 32 the row pair is
+// combined across lane distance 16.
+template <int W, typename Op>
+__device__ __forceinline__ uint32_t sub_reduce(uint32_t v, Op op) {
+  static_assert(W == 16 || W == 32, "sub-group width");
+  v = op(v, dpp<0x121>(v, v));
+  v = op(v, dpp<0x122>(v, v));
+  v = op(v, dpp<0x124>(v, v));
+  v = op(v, dpp<0x128>(v, v));
+  if constexpr (W == 32) v = op(v, (uint32_t)__shfl_xor((int)v, 16, kWave));
   return v;
+}
+template <int W>
+__device__ __forceinline__ float sub_min(float v) {
+  return __uint_as_float(sub_reduce<W>(__float_as_uint(v), [](uint32_t a, uint32_t b) {
+    return __float_as_uint(fminf(__uint_as_float(a), __uint_as_float(b)));
+  }));
+}
+template <int W>
+__device__ __forceinline__ float sub_max(float v) {
+  return __uint_as_float(sub_reduce<W>(__float_as_uint(v), [](uint32_t a, uint32_t b) {
+    return __float_as_uint(fmaxf(__uint_as_float(a), __uint_as_float(b)));
+  }));
+}
+template <int W>
+__device__ __forceinline__ uint32_t sub_sum(uint32_t v) {
+  return sub_reduce<W>(v, [](uint32_t a, uint32_t b) { return a + b; });
+}
+// inclusive scan within each group of W lanes
+template <int W>
+__device__ __forceinline__ uint32_t sub_incl_scan(uint32_t v) {
+  v += dpp<0x111>(0u, v);
+  v += dpp<0x112>(0u, v);
+  v += dpp<0x114>(0u, v);
+  v += dpp<0x118>(0u, v);
+  if constexpr (W == 32) v += dpp<0x142, 0xA>(0u, v);
+  return v;
+}
+// wave maximum of uint32, wave-uniform
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  auto op = [](uint32_t a, uint32_t b) { return a > b ? a : b; };
+  v = op(v, dpp<0x121>(v, v));
+  v = op(v, dpp<0x122>(v, v));
+  v = op(v, dpp<0x124>(v, v));
+  v = op(v, dpp<0x128>(v, v));
+  v = op(v, dpp<0x142, 0xA>(v, v));
+  v = op(v, dpp<0x143, 0xC>(v, v));
+  return __builtin_amdgcn_readlane(v, kWave - 1);
 }
 
 // Workgroups are dispatched round-robin over the XCDs (workgroup b runs on
@@ -557,7 +639,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
 // have at most tq queries and span at most `group` dim-0 boxes.
 namespace {
 template <typename KeyT>
-__device__ __forceinline__ bool tile_start(const DJob& J, const KeyT* keys, uint32_t p, uint32_t tq) {
+__device__ __forceinline__ bool tile_start(const DJob& J, const KeyT* keys, const uint32_t* cstart, uint32_t p,
+                                           uint32_t tq) {
   if (p == 0 || (p % tq) == 0) return true;
   if (J.flags & kJobBrute) return false;
   // key width arithmetic (32-bit division for 32-bit keys)
@@ -565,7 +648,28 @@ __device__ __forceinline__ bool tile_start(const DJob& J, const KeyT* keys, uint
   const KeyT nb0 = (KeyT)J.nb[0];
   const KeyT ra = a / nb0, rb = b / nb0;
   if (ra != rb) return true;
-  return (uint32_t)(a - ra * nb0) / J.group != (uint32_t)(b - rb * nb0) / J.group;
+  const uint32_t ga = (uint32_t)(a - ra * nb0) / J.group, gb = (uint32_t)(b - rb * nb0) / J.group;
+  if (ga == gb) return false;
+#if ENT_MERGE_LEVELS > 0
+  // Sparse stretches of a row: groups merge into one tile when the smallest
+  // aligned block of 2^L groups holding both sides of the step (L <=
+  // ENT_MERGE_LEVELS) has at most tq points, counted from the cell-start
+  // table.  Every tile between two kept steps then lies inside one such
+  // block (a run crossing a block edge of level L steps over a pair whose
+  // common block is above L).
+  if (J.cbase != kNoTable) {
+    const int L = 32 - __builtin_clz(ga ^ gb);
+    if (L <= ENT_MERGE_LEVELS) {
+      const uint64_t row0 = (uint64_t)rb * J.nb[0];
+      const uint64_t g_lo = (uint64_t)((gb >> L) << L) * J.group;
+      const uint64_t g_end = g_lo + ((uint64_t)J.group << L);
+      const uint64_t g_hi = g_end < J.nb[0] ? g_end : J.nb[0];
+      const uint32_t* st = cstart + J.cbase;
+      if (st[row0 + g_hi] - st[row0 + g_lo] <= tq) return false;
+    }
+  }
+#endif
+  return true;
 }
 }  // namespace
 
@@ -574,6 +678,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(const DJob* __restrict__ 
                                                        const uint2* __restrict__ chunks,
                                                        uint32_t n_chunks,
                                                        const KeyT* __restrict__ keys,
+                                                       const uint32_t* __restrict__ cstart,
                                                        uint32_t tq, uint32_t* __restrict__ cnt) {
   const uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (c >= n_chunks) return;
@@ -583,7 +688,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(const DJob* __restrict__ 
   uint32_t total = 0;
   for (int i = 0; i < kChunkItems; ++i) {
     uint32_t p = ch.y + i * kWave + l;
-    bool f = (p < J.n) && tile_start(J, keys, p, tq);
+    bool f = (p < J.n) && tile_start(J, keys, cstart, p, tq);
     total += (uint32_t)__popcll(__ballot(f));
   }
   if (l == 0) cnt[c] = total;
@@ -594,6 +699,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_write(const DJob* __restrict__ 
                                                        const uint2* __restrict__ chunks,
                                                        uint32_t n_chunks,
                                                        const KeyT* __restrict__ keys,
+                                                       const uint32_t* __restrict__ cstart,
                                                        uint32_t tq,
                                                        const uint32_t* __restrict__ offs,
                                                        DTile* __restrict__ tiles) {
@@ -606,14 +712,14 @@ __global__ __launch_bounds__(kBlock) void k_tile_write(const DJob* __restrict__ 
   uint32_t o = offs[c];
   for (int i = 0; i < kChunkItems; ++i) {
     uint32_t p = ch.y + i * kWave + l;
-    bool f = (p < J.n) && tile_start(J, keys, p, tq);
+    bool f = (p < J.n) && tile_start(J, keys, cstart, p, tq);
     uint64_t b = __ballot(f);
     if (f) {
       DTile t;
       t.job = ch.x;
       t.p0 = p;
       t.p1 = 0;
-      t.pad = 0;
+      t.pack = 0;
       tiles[o + __popcll(b & lt)] = t;
     }
     o += (uint32_t)__popcll(b);
@@ -632,6 +738,51 @@ __global__ void k_tile_ends(const DJob* __restrict__ jobs, DTile* __restrict__ t
       if (b.job == a.job) end = b.p0;
     }
     tiles[t].p1 = end;
+  }
+}
+
+// Packed tiles: each aligned quad of tiles 4q .. 4q+3 from one job (d >= 2)
+// with at most 16 queries each becomes one packed tile of four sub-tiles;
+// otherwise each aligned pair from one job with at most 32 queries each
+// becomes a packed tile of two.  The first tile of a pack takes the whole
+// span and the sub-tile offsets; the others are marked kTileSkip.  One
+// thread per quad rewrites its four tiles in place.
+__global__ void k_tile_pack(const DJob* __restrict__ jobs, DTile* __restrict__ tiles,
+                            const uint32_t* __restrict__ total) {
+  const uint32_t T = *total;
+  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; 4 * q < T;
+       q += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t t0 = (uint32_t)(4 * q);
+    const uint32_t n = T - t0 < 4u ? T - t0 : 4u;
+    DTile t[4];
+    for (uint32_t i = 0; i < n; ++i) t[i] = tiles[t0 + i];
+    auto fits = [&](uint32_t a, uint32_t cnt, uint32_t lim) {
+      if (a + cnt > n) return false;
+      const DJob& J = jobs[t[a].job];
+      if (J.d < 2 || (J.flags & kJobBrute)) return false;
+      for (uint32_t i = a; i < a + cnt; ++i)
+        if (t[i].job != t[a].job || t[i].p1 - t[i].p0 > lim) return false;
+      return true;
+    };
+    auto pack = [&](uint32_t a, uint32_t cnt) {
+      DTile h = t[a];
+      uint32_t code = cnt - 1;
+      for (uint32_t i = 1; i < cnt; ++i) code |= (t[a + i].p0 - h.p0) << (8 * i);
+      h.p1 = t[a + cnt - 1].p1;
+      h.pack = code;
+      tiles[t0 + a] = h;
+      for (uint32_t i = 1; i < cnt; ++i) {
+        DTile sk = t[a + i];
+        sk.pack = kTileSkip;
+        tiles[t0 + a + i] = sk;
+      }
+    };
+    if (fits(0, 4, 16)) {
+      pack(0, 4);
+    } else {
+      if (fits(0, 2, 32)) pack(0, 2);
+      if (fits(2, 2, 32)) pack(2, 2);
+    }
   }
 }
 
@@ -656,6 +807,7 @@ __device__ __forceinline__ uint32_t lower_bound_key(const KeyT* a, uint32_t n, u
 
 struct WaveLds {
   float4 cand[kWave];        // staged candidates (local coordinates)
+  uint2 rng[kWave];          // packed tiles: candidate ranges (first flattened index, first position)
 };
 
 // Conservative box range [lo, hi] covering [a - h, b + h] along one dim.
@@ -689,16 +841,17 @@ __device__ __forceinline__ void kahan_add(float& S, float& C, float x) {
 // Accumulate one staged chunk of 64 candidates into the Q queries of a lane.
 // RAW: stage holds raw coordinates, u = (xq - xc) * hinv (plain path).
 // else: stage holds (x - o) * hinv, u = sq - sc.
-template <int D, int Q, bool RAW, bool TRI = false>
-__device__ __forceinline__ void accumulate_chunk(const WaveLds& L, const float (&sq)[3][Q],
+// NC: candidates staged at c4 (64 for a whole chunk, 16 or 32 for the
+// sub-tile of a packed tile, whose lanes read their own sub-tile's stage).
+template <int D, int Q, bool RAW, bool TRI = false, int NC = kWave>
+__device__ __forceinline__ void accumulate_chunk(const float4* c4, const float (&sq)[3][Q],
                                                  const float (&hinv)[3], float (&S)[Q],
                                                  float (&C)[Q]) {
-  const float4* c4 = L.cand;
   float part[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) part[q] = 0.0f;
   constexpr int kPer4 = (D == 1) ? 4 : (D == 2 ? 2 : 1);
-  constexpr int kReads = kWave / kPer4;  // float4 LDS reads per chunk
+  constexpr int kReads = NC / kPer4;  // float4 LDS reads per chunk
 
   // the candidates of one float4 read against the lane's queries
   auto consume = [&](const float4 v, const int g) {
@@ -744,8 +897,8 @@ __device__ __forceinline__ void accumulate_chunk(const WaveLds& L, const float (
         }
       }
     }
-    // compensated merge every 32 candidates
-    if (((g + 1) * kPer4) % 32 == 0) {
+    // compensated merge every 32 candidates (and at the end of the stage)
+    if (((g + 1) * kPer4) % 32 == 0 || g + 1 == kReads) {
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         kahan_add(S[q], C[q], part[q]);
@@ -852,7 +1005,7 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
   // flattened candidate index space over the ranges: lane k < n_ranges
   // holds range k as (first sorted position, first flattened index)
   const uint32_t incl = wave_incl_scan(my_len);
-  const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, kWave - 1, kWave));
+  const uint32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
   const uint32_t my_ps = incl - my_len;
 
   float sq[3][Q];
@@ -865,19 +1018,29 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
 #pragma unroll
   for (int q = 0; q < Q; ++q) { S[q] = 0.0f; C[q] = 0.0f; }
 
-  // candidate loader: flattened index f -> raw coordinates (converted to
-  // local coordinates only when staged, so the load's latency overlaps the
-  // previous chunk's pair loop).  The range of f is the last range whose
-  // first flattened index is <= f (empty ranges share their successor's
-  // start and are overridden by it); ranges are read from their lanes.
-  auto load_cand = [&](uint32_t f, float* x) {
+  // candidate loader for the chunk of flattened indices [cb, cb + 64): lane l
+  // loads index cb + l as raw coordinates (converted to local coordinates
+  // only when staged, so the load's latency overlaps the previous chunk's
+  // pair loop).  Range k (read from lane k) covers [ps_k, ps_k + len_k);
+  // the range of an index is the last one starting at or before it (empty
+  // ranges share their successor's start and are overridden by it).  Chunks
+  // come in increasing order, so the scan resumes at kc, the last range
+  // starting at or before the previous chunk, and stops at the first range
+  // starting past this chunk: a chunk visits the ranges that start inside it.
+  uint32_t kc = 0;
+  auto load_cand = [&](uint32_t cb, float* x) {
+    const uint32_t f = cb + (uint32_t)l;
+    const uint32_t cend = cb + kWave;
+    const bool cend_wraps = cend < cb;
+    uint32_t off = 0;  // sorted position - flattened index, mod 2^32
+    for (uint32_t k = kc; k < n_ranges; ++k) {
+      const uint32_t ps_k = __builtin_amdgcn_readlane(my_ps, k);
+      if (!cend_wraps && ps_k >= cend) break;
+      const uint32_t lo_k = __builtin_amdgcn_readlane(my_lo, k);
+      if (ps_k <= cb) kc = k;
+      if (f >= ps_k) off = lo_k - ps_k;
+    }
     if (f < total) {
-      uint32_t off = 0;  // sorted position - flattened index, mod 2^32
-      for (uint32_t k = 0; k < n_ranges; ++k) {
-        const uint32_t ps_k = __builtin_amdgcn_readlane(my_ps, k);
-        const uint32_t lo_k = __builtin_amdgcn_readlane(my_lo, k);
-        if (f >= ps_k) off = lo_k - ps_k;
-      }
       const uint32_t p = f + off;
 #pragma unroll
       for (int e = 0; e < D; ++e) x[e] = src[e][p];
@@ -896,16 +1059,16 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
     wave_sync();
     stage_store<D>(L, l, s_c);
     wave_sync();
-    if (base + ahead < total) load_cand(base + ahead + (uint32_t)l, buf);
+    if (base + ahead < total) load_cand(base + ahead, buf);
     if constexpr (D == 1) {
-      if (tri) accumulate_chunk<1, Q, RAW, true>(L, sq, hinv, S, C);
-      else accumulate_chunk<1, Q, RAW, false>(L, sq, hinv, S, C);
+      if (tri) accumulate_chunk<1, Q, RAW, true>(L.cand, sq, hinv, S, C);
+      else accumulate_chunk<1, Q, RAW, false>(L.cand, sq, hinv, S, C);
     } else {
-      accumulate_chunk<D, Q, RAW>(L, sq, hinv, S, C);
+      accumulate_chunk<D, Q, RAW>(L.cand, sq, hinv, S, C);
     }
   };
   float nxt[3];
-  load_cand((uint32_t)l, nxt);
+  load_cand(0u, nxt);
   for (uint32_t base = 0; base < total; base += kWave) chunk(base, nxt, kWave);
   wave_sync();
 
@@ -918,6 +1081,143 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
     }
   }
   return total;
+}
+
+// Densities of a packed tile: G sub-tiles of at most W = 64 / G queries,
+// one query per lane, lanes [s W, s W + W) on sub-tile s.  Each sub-tile has
+// its own bounding box, candidate ranges (held in the wave's LDS range
+// table) and stage slice of W candidates; the wave steps through the
+// longest sub-tile's candidates W at a time.  Returns the executed pairs.
+template <int D, int G, typename KeyT>
+__device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __restrict__ scoord,
+                                 const KeyT* __restrict__ keys, const uint32_t* __restrict__ cstart,
+                                 const uint32_t* __restrict__ idx, double* __restrict__ dens, WaveLds& L,
+                                 uint64_t& lane_slots) {
+  constexpr int W = kWave / G;
+  constexpr int kPer4 = D == 2 ? 2 : 1;
+  const int l = lane_id();
+  const int s = l / W, j = l % W;
+  const uint32_t n = J.n;
+  const float* src[3];
+#pragma unroll
+  for (int e = 0; e < D; ++e) src[e] = scoord + 3 * J.pbase + (uint64_t)e * n;
+  float hinv[3] = {J.hinv[0], J.hinv[1], J.hinv[2]};
+
+  // this lane's sub-tile and query
+  const uint32_t span = T.p1 - T.p0;
+  const uint32_t b0 = s == 0 ? 0u : (T.pack >> (8 * s)) & 0xFFu;
+  const uint32_t b1 = s + 1 == G ? span : (T.pack >> (8 * (s + 1))) & 0xFFu;
+  const uint32_t nq = b1 - b0;
+  const bool qv = (uint32_t)j < nq;
+  const uint32_t p = T.p0 + b0 + (qv ? (uint32_t)j : 0u);
+  float xq[3];
+#pragma unroll
+  for (int e = 0; e < D; ++e) xq[e] = src[e][p];
+
+  // sub-tile bounding box (idle lanes repeat the sub-tile's first query);
+  // its lower corner is the sub-tile's origin
+  float o[3] = {0.f, 0.f, 0.f};
+  uint32_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+  for (int e = 0; e < D; ++e) {
+    o[e] = sub_min<W>(xq[e]);
+    const float mx = sub_max<W>(xq[e]);
+    box_range(o[e], mx, J.h[e], J.mn[e], J.bs[e], J.nb[e], lo[e], hi[e]);
+  }
+  const uint32_t n1 = hi[1] - lo[1] + 1;
+  const uint32_t n2 = (D == 3) ? (hi[2] - lo[2] + 1) : 1u;
+  uint32_t nr = n1 * n2;
+  uint32_t my_lo = 0, my_len = 0;
+  const bool tab = J.cbase != kNoTable;
+  const KeyT* kk = keys + J.pbase;
+  if (nr <= (uint32_t)W) {
+    if ((uint32_t)j < nr) {
+      const uint32_t r1 = lo[1] + (uint32_t)j % n1;
+      const uint32_t r2 = (D == 3) ? lo[2] + (uint32_t)j / n1 : 0u;
+      const uint64_t row = (uint64_t)r2 * J.nb[1] + r1;
+      const uint64_t klo = row * J.nb[0] + lo[0];
+      const uint64_t khi = row * J.nb[0] + hi[0];
+      uint32_t a, b;
+      if (tab) {
+        a = cstart[J.cbase + klo];
+        b = cstart[J.cbase + khi + 1];
+      } else {
+        a = lower_bound_key(kk, n, klo);
+        b = a + lower_bound_key(kk + a, n - a, khi + 1);
+      }
+      my_lo = a;
+      my_len = b - a;
+    }
+  } else {
+    // more candidate rows than lanes: one range over every row between the
+    // first and last candidate row (a superset)
+    if (j == 0) {
+      const uint64_t row_a = (uint64_t)lo[2] * J.nb[1] + lo[1];
+      const uint64_t row_b = (uint64_t)hi[2] * J.nb[1] + hi[1];
+      const uint32_t a = tab ? cstart[J.cbase + row_a * J.nb[0]] : lower_bound_key(kk, n, row_a * J.nb[0]);
+      const uint32_t b =
+          tab ? cstart[J.cbase + (row_b + 1) * J.nb[0]] : lower_bound_key(kk, n, (row_b + 1) * J.nb[0]);
+      my_lo = a;
+      my_len = b - a;
+    }
+    nr = 1;
+  }
+  // flattened candidate index space of the sub-tile; the range table goes
+  // to LDS, entry s W + k = (first flattened index, first position) of range k
+  const uint32_t incl = sub_incl_scan<W>(my_len);
+  const uint32_t total = sub_sum<W>(my_len);
+  wave_sync();
+  L.rng[l] = make_uint2(incl - my_len, my_lo);
+  wave_sync();
+  const uint32_t steps = wave_max_u32((total + W - 1) / W);
+
+  float sq[3][1];
+#pragma unroll
+  for (int e = 0; e < D; ++e) sq[e][0] = (xq[e] - o[e]) * hinv[e];
+  float S[1] = {0.0f}, C[1] = {0.0f};
+
+  // candidate f of the sub-tile: range k is the last one starting at or
+  // before f; steps come in increasing order, so kc only moves forward
+  uint32_t kc = 0;
+  const uint2* rng = L.rng + s * W;
+  auto load_cand = [&](uint32_t step, float* x) {
+    const uint32_t f = step * W + (uint32_t)j;
+    if (f < total) {
+      while (kc + 1 < nr && rng[kc + 1].x <= f) ++kc;
+      const uint2 r = rng[kc];
+      const uint32_t pos = f - r.x + r.y;
+#pragma unroll
+      for (int e = 0; e < D; ++e) x[e] = src[e][pos];
+    } else {
+#pragma unroll
+      for (int e = 0; e < D; ++e) x[e] = kSentinel;
+    }
+  };
+  const float4* stage = L.cand + s * (W / kPer4);
+  float nxt[3];
+  load_cand(0u, nxt);
+  for (uint32_t b = 0; b < steps; ++b) {
+    float s_c[3];
+#pragma unroll
+    for (int e = 0; e < D; ++e) s_c[e] = (nxt[e] - o[e]) * hinv[e];
+    wave_sync();
+    stage_store<D>(L, l, s_c);
+    wave_sync();
+    if (b + 1 < steps) load_cand(b + 1, nxt);
+    accumulate_chunk<D, 1, false, false, W>(stage, sq, hinv, S, C);
+  }
+  wave_sync();
+
+  if (qv) {
+    const uint32_t k = idx[J.pbase + p];
+    dens[J.out_off + k] = ((double)S[0] - (double)C[0]) * J.norm;
+  }
+  lane_slots += (uint64_t)kWave * W * steps;
+  uint64_t pr = 0;
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+    pr += (uint64_t)__builtin_amdgcn_readlane(nq, g * W) * __builtin_amdgcn_readlane(total, g * W);
+  return pr;
 }
 
 // Queries per lane chosen per tile: the smallest Q <= QMAX whose 64*Q lane
@@ -983,8 +1283,22 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
       if (i >= hi - lo) break;
       const uint32_t t = lo + i;
       const DTile T = tiles[t];
+      const uint32_t pk = __builtin_amdgcn_readfirstlane(T.pack);
+      if (pk == kTileSkip) continue;
       const DJob J = jobs[T.job];
       const uint32_t d = __builtin_amdgcn_readfirstlane(J.d);
+      if constexpr (!RAW) {
+        if (pk & 3u) {
+          const bool four = (pk & 3u) == 3u;
+          if (d == 2)
+            acc[1] += four ? process_pack<2, 4, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots)
+                           : process_pack<2, 2, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots);
+          else
+            acc[2] += four ? process_pack<3, 4, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots)
+                           : process_pack<3, 2, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots);
+          continue;
+        }
+      }
       if (d == 1) run_tile<1, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[0], slots);
       else if (d == 2) run_tile<2, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[1], slots);
       else run_tile<3, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[2], slots);
@@ -1117,23 +1431,24 @@ void launch_gather_sorted(hipStream_t s, const DJob* jobs, const uint2* chunks, 
 }
 
 void launch_tile_count(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const void* keys, uint32_t tq, uint32_t* cnt) {
+                       const void* keys, const uint32_t* cstart, uint32_t tq, uint32_t* cnt) {
   if (wide)
     hipLaunchKernelGGL(k_tile_count<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint64_t*)keys, tq, cnt);
+                       n_chunks, (const uint64_t*)keys, cstart, tq, cnt);
   else
     hipLaunchKernelGGL(k_tile_count<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint32_t*)keys, tq, cnt);
+                       n_chunks, (const uint32_t*)keys, cstart, tq, cnt);
 }
 
 void launch_tile_write(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const void* keys, uint32_t tq, const uint32_t* offs, DTile* tiles) {
+                       const void* keys, const uint32_t* cstart, uint32_t tq, const uint32_t* offs,
+                       DTile* tiles) {
   if (wide)
     hipLaunchKernelGGL(k_tile_write<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint64_t*)keys, tq, offs, tiles);
+                       n_chunks, (const uint64_t*)keys, cstart, tq, offs, tiles);
   else
     hipLaunchKernelGGL(k_tile_write<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint32_t*)keys, tq, offs, tiles);
+                       n_chunks, (const uint32_t*)keys, cstart, tq, offs, tiles);
 }
 
 void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
@@ -1142,6 +1457,17 @@ void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint3
   if (blocks > 8192) blocks = 8192;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_tile_ends, dim3((unsigned)blocks), dim3(kBlock), 0, s, jobs, tiles, total);
+}
+void launch_tile_pack(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
+                      uint64_t max_tiles) {
+#if ENT_TILE_PACK
+  uint64_t blocks = (max_tiles / 4 + kBlock) / kBlock;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_tile_pack, dim3((unsigned)blocks), dim3(kBlock), 0, s, jobs, tiles, total);
+#else
+  (void)s; (void)jobs; (void)tiles; (void)total; (void)max_tiles;
+#endif
 }
 
 // Kernel variants: RAW (plain path, keys unused) or cell lists with 32- or
