@@ -927,9 +927,11 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
   float hinv[3] = {J.hinv[0], J.hinv[1], J.hinv[2]};
   const bool tri = (J.flags & kJobTri) != 0;
 
-  // queries
+  // queries, and their output slots (loaded now so the tile's end does not
+  // wait for another round trip)
   float xq[3][Q];
   bool qv[Q];
+  uint32_t slot[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     uint32_t p = T.p0 + q * kWave + l;
@@ -937,6 +939,7 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
     uint32_t pp = qv[q] ? p : T.p0;
 #pragma unroll
     for (int e = 0; e < D; ++e) xq[e][q] = src[e][pp];
+    slot[q] = RAW ? pp : idx[J.pbase + pp];
   }
 
   float o[3] = {0.f, 0.f, 0.f};
@@ -1074,11 +1077,7 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
 
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    if (qv[q]) {
-      uint32_t p = T.p0 + q * kWave + l;
-      uint32_t k = RAW ? p : idx[J.pbase + p];
-      dens[J.out_off + k] = ((double)S[q] - (double)C[q]) * J.norm;
-    }
+    if (qv[q]) dens[J.out_off + slot[q]] = ((double)S[q] - (double)C[q]) * J.norm;
   }
   return total;
 }
@@ -1113,6 +1112,7 @@ __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __r
   float xq[3];
 #pragma unroll
   for (int e = 0; e < D; ++e) xq[e] = src[e][p];
+  const uint32_t slot = idx[J.pbase + p];
 
   // sub-tile bounding box (idle lanes repeat the sub-tile's first query);
   // its lower corner is the sub-tile's origin
@@ -1208,10 +1208,7 @@ __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __r
   }
   wave_sync();
 
-  if (qv) {
-    const uint32_t k = idx[J.pbase + p];
-    dens[J.out_off + k] = ((double)S[0] - (double)C[0]) * J.norm;
-  }
+  if (qv) dens[J.out_off + slot] = ((double)S[0] - (double)C[0]) * J.norm;
   lane_slots += (uint64_t)kWave * W * steps;
   uint64_t pr = 0;
 #pragma unroll
