@@ -238,15 +238,28 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(const DJob* __restrict__ j
   const uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (c >= n_chunks) return;
   const uint2 ch = chunks[c];
-  const DJob J = jobs[ch.x];
+  const DJob& J = jobs[ch.x];
   const int l = lane_id();
-  for (int i = 0; i < kChunkItems; ++i) {
-    uint32_t k = ch.y + i * kWave + l;
-    if (k < J.n) {
-      float v[3] = {0.f, 0.f, 0.f};
-      for (uint32_t e = 0; e < J.d; ++e) v[e] = X[J.off[e] + k];
-      keys[J.pbase + k] = (KeyT)cell_key(J, v);
-      idx[J.pbase + k] = k;
+  const uint32_t d = __builtin_amdgcn_readfirstlane(J.d);
+  const uint32_t last = J.n - 1;
+  // coordinates of the chunk loaded in batches of 8 per lane before use
+  for (int i0 = 0; i0 < kChunkItems; i0 += 8) {
+    float v[8][3];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const uint32_t k = ch.y + (uint32_t)(i0 + i) * kWave + l;
+      const uint32_t kc = k < last ? k : last;
+      v[i][0] = X[J.off[0] + kc];
+      v[i][1] = d >= 2 ? X[J.off[1] + kc] : 0.f;
+      v[i][2] = d >= 3 ? X[J.off[2] + kc] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const uint32_t k = ch.y + (uint32_t)(i0 + i) * kWave + l;
+      if (k < J.n) {
+        keys[J.pbase + k] = (KeyT)cell_key(J, v[i]);
+        idx[J.pbase + k] = k;
+      }
     }
   }
 }
@@ -281,12 +294,18 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const DJob* __restrict__ 
   if (c >= n_chunks) return;
   const uint2 ch = chunks[c];
   const DJob& J = jobs[ch.x];
+  // all of the chunk's keys are loaded before the first use
+  KeyT kr[kChunkItems];
+  const uint32_t last = J.n - 1;
+#pragma unroll
   for (int i = 0; i < kChunkItems; ++i) {
-    uint32_t k = ch.y + i * kWave + l;
-    if (k < J.n) {
-      uint32_t dg = (uint32_t)(keys[J.pbase + k] >> shift) & (kRadixBins - 1);
-      atomicAdd(&bins[w][dg], 1u);
-    }
+    const uint32_t k = ch.y + i * kWave + l;
+    kr[i] = keys[J.pbase + (k < last ? k : last)];
+  }
+#pragma unroll
+  for (int i = 0; i < kChunkItems; ++i) {
+    const uint32_t k = ch.y + i * kWave + l;
+    if (k < J.n) atomicAdd(&bins[w][(uint32_t)(kr[i] >> shift) & (kRadixBins - 1)], 1u);
   }
   wave_sync();
   const uint32_t c0 = chunk_first[ch.x];
@@ -369,6 +388,43 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
 // With GATHER (last pass) the full cell key is recomputed from the gathered
 // coordinates, so a pass over a partial key (the two-stage sort) also ends
 // with full keys.
+// Output phase of a gathering scatter pass for jobs of dimension D: staged
+// points leave in digit order, kB per lane at a time (positions from LDS,
+// then the coordinate gathers, then the stores).
+template <int D, typename KeyT>
+__device__ __forceinline__ void scatter_gather_out(const DJob& J, const KeyT* skey, const uint32_t* sid,
+                                                   const uint32_t* delta, uint32_t nv, int shift,
+                                                   const float* __restrict__ X, KeyT* __restrict__ keys_out,
+                                                   uint32_t* __restrict__ idx_out, float* __restrict__ scoord,
+                                                   int l) {
+  constexpr int kB = 8;
+  for (uint32_t j0 = 0; j0 < nv; j0 += kB * kWave) {
+    uint32_t id[kB], dst[kB];
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const uint32_t j = j0 + (uint32_t)i * kWave + l;
+      const uint32_t jj = j < nv ? j : 0u;
+      id[i] = sid[jj];
+      dst[i] = delta[(uint32_t)(skey[jj] >> shift) & (kRadixBins - 1)] + jj;
+    }
+    float v[kB][3];
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      v[i][0] = v[i][1] = v[i][2] = 0.0f;
+#pragma unroll
+      for (int e = 0; e < D; ++e) v[i][e] = X[J.off[e] + id[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      if (j0 + (uint32_t)i * kWave + l >= nv) continue;
+      idx_out[dst[i]] = id[i];
+      const uint64_t p = dst[i] - J.pbase;
+#pragma unroll
+      for (int e = 0; e < D; ++e) scoord[3 * J.pbase + (uint64_t)e * J.n + p] = v[i][e];
+      keys_out[dst[i]] = (KeyT)cell_key(J, v[i]);
+    }
+  }
+}
 template <typename KeyT, bool GATHER>
 __global__ __launch_bounds__(kWave) void k_radix_scatter_lds(
     const DJob* __restrict__ jobs, const uint2* __restrict__ chunks,
@@ -442,20 +498,16 @@ __global__ __launch_bounds__(kWave) void k_radix_scatter_lds(
     }
     wave_sync();
   }
-  for (uint32_t j = l; j < nv; j += kWave) {
-    const KeyT key = skey[j];
-    const uint32_t id = sid[j];
-    const uint32_t dst = delta[(uint32_t)(key >> shift) & (kRadixBins - 1)] + j;
-    idx_out[dst] = id;
-    if constexpr (GATHER) {
-      const uint64_t p = dst - J.pbase;
-      float v[3] = {0.f, 0.f, 0.f};
-      for (uint32_t e = 0; e < J.d; ++e) {
-        v[e] = X[J.off[e] + id];
-        scoord[3 * J.pbase + (uint64_t)e * J.n + p] = v[e];
-      }
-      keys_out[dst] = (KeyT)cell_key(J, v);
-    } else {
+  if constexpr (GATHER) {
+    const uint32_t d = __builtin_amdgcn_readfirstlane(J.d);
+    if (d == 1) scatter_gather_out<1>(J, skey, sid, delta, nv, shift, X, keys_out, idx_out, scoord, l);
+    else if (d == 2) scatter_gather_out<2>(J, skey, sid, delta, nv, shift, X, keys_out, idx_out, scoord, l);
+    else scatter_gather_out<3>(J, skey, sid, delta, nv, shift, X, keys_out, idx_out, scoord, l);
+  } else {
+    for (uint32_t j = l; j < nv; j += kWave) {
+      const KeyT key = skey[j];
+      const uint32_t dst = delta[(uint32_t)(key >> shift) & (kRadixBins - 1)] + j;
+      idx_out[dst] = sid[j];
       keys_out[dst] = key;
     }
   }
@@ -470,6 +522,52 @@ __global__ __launch_bounds__(kWave) void k_radix_scatter_lds(
 // point order -- the same permutation as one sort by the full key.
 // Writes (r, point) at the job's own positions.  With scoord (no pass over r
 // follows: every r is 0) it writes the full key and the sorted coordinates.
+// One chunk of k_major_keys for jobs of dimension D, in batches of kB
+// points: base positions first, then the coordinate gathers, then the
+// stores, so a batch's loads are in flight together.
+template <int D, bool FINAL, typename KeyT>
+__device__ __forceinline__ void major_keys_chunk(const DJob& J, uint32_t y, const uint32_t* __restrict__ bi,
+                                                 const float* __restrict__ X, KeyT* __restrict__ keys,
+                                                 uint32_t* __restrict__ idx, float* __restrict__ scoord, int l) {
+  constexpr int kB = 16;
+  constexpr int E0 = FINAL ? 0 : 1;  // without coordinates only the dims above 0 are needed
+  const uint32_t last = J.n - 1;
+#pragma unroll
+  for (int i0 = 0; i0 < kChunkItems; i0 += kB) {
+    uint32_t id[kB];
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const uint32_t k = y + (uint32_t)(i0 + i) * kWave + l;
+      id[i] = bi[k < last ? k : last];
+    }
+    float v[kB][3];
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      v[i][0] = v[i][1] = v[i][2] = 0.0f;
+#pragma unroll
+      for (int e = E0; e < D; ++e) v[i][e] = X[J.off[e] + id[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const uint32_t k = y + (uint32_t)(i0 + i) * kWave + l;
+      if (k >= J.n) continue;
+      idx[J.pbase + k] = id[i];
+      if constexpr (FINAL) {
+#pragma unroll
+        for (int e = 0; e < D; ++e) scoord[3 * J.pbase + (uint64_t)e * J.n + k] = v[i][e];
+        keys[J.pbase + k] = (KeyT)cell_key(J, v[i]);
+      } else {
+        uint64_t r = 0;
+        if constexpr (D >= 2) {
+          r = box_of(v[i][1], J.mn[1], J.bs[1], J.nb[1]);
+          if constexpr (D == 3) r += (uint64_t)box_of(v[i][2], J.mn[2], J.bs[2], J.nb[2]) * J.nb[1];
+        }
+        keys[J.pbase + k] = (KeyT)r;
+      }
+    }
+  }
+}
+
 template <typename KeyT>
 __global__ __launch_bounds__(kBlock) void k_major_keys(const DJob* __restrict__ jobs,
                                                        const uint2* __restrict__ chunks,
@@ -486,27 +584,15 @@ __global__ __launch_bounds__(kBlock) void k_major_keys(const DJob* __restrict__ 
   const DJob J = jobs[ch.x];
   const uint32_t* bi = bidx + bbase[ch.x];
   const int l = lane_id();
-  for (int i = 0; i < kChunkItems; ++i) {
-    const uint32_t k = ch.y + i * kWave + l;
-    if (k < J.n) {
-      const uint32_t id = bi[k];
-      idx[J.pbase + k] = id;
-      if (scoord) {
-        float v[3] = {0.f, 0.f, 0.f};
-        for (uint32_t e = 0; e < J.d; ++e) {
-          v[e] = X[J.off[e] + id];
-          scoord[3 * J.pbase + (uint64_t)e * J.n + k] = v[e];
-        }
-        keys[J.pbase + k] = (KeyT)cell_key(J, v);
-      } else {
-        uint64_t r = 0;
-        if (J.d >= 2) {
-          r = box_of(X[J.off[1] + id], J.mn[1], J.bs[1], J.nb[1]);
-          if (J.d == 3) r += (uint64_t)box_of(X[J.off[2] + id], J.mn[2], J.bs[2], J.nb[2]) * J.nb[1];
-        }
-        keys[J.pbase + k] = (KeyT)r;
-      }
-    }
+  const uint32_t d = __builtin_amdgcn_readfirstlane(J.d);
+  if (scoord) {
+    if (d == 1) major_keys_chunk<1, true>(J, ch.y, bi, X, keys, idx, scoord, l);
+    else if (d == 2) major_keys_chunk<2, true>(J, ch.y, bi, X, keys, idx, scoord, l);
+    else major_keys_chunk<3, true>(J, ch.y, bi, X, keys, idx, scoord, l);
+  } else {
+    if (d == 1) major_keys_chunk<1, false>(J, ch.y, bi, X, keys, idx, scoord, l);
+    else if (d == 2) major_keys_chunk<2, false>(J, ch.y, bi, X, keys, idx, scoord, l);
+    else major_keys_chunk<3, false>(J, ch.y, bi, X, keys, idx, scoord, l);
   }
 }
 
@@ -569,14 +655,16 @@ __global__ __launch_bounds__(kBlock) void k_gather_sorted(const DJob* __restrict
   const uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (c >= n_chunks) return;
   const uint2 ch = chunks[c];
-  const DJob J = jobs[ch.x];
+  const DJob& J = jobs[ch.x];
   const int l = lane_id();
+  const uint32_t d = __builtin_amdgcn_readfirstlane(J.d);
   for (int i = 0; i < kChunkItems; ++i) {
     uint32_t p = ch.y + i * kWave + l;
     if (p < J.n) {
       uint32_t k = idx[J.pbase + p];
-      for (uint32_t e = 0; e < J.d; ++e)
-        scoord[3 * J.pbase + (uint64_t)e * J.n + p] = X[J.off[e] + k];
+#pragma unroll
+      for (uint32_t e = 0; e < 3; ++e)
+        if (e < d) scoord[3 * J.pbase + (uint64_t)e * J.n + p] = X[J.off[e] + k];
     }
   }
 }
@@ -584,6 +672,21 @@ __global__ __launch_bounds__(kBlock) void k_gather_sorted(const DJob* __restrict
 // ------------------------------------------------------------ cell starts
 // start[cbase + c] = first sorted position whose key >= c, for c in
 // [0, cells]; each position fills the gap of empty cells before its own key.
+// Keys of points p and p - 1 (clamped to the job) for items [I0, I0 + N)
+// of a chunk, all loads issued before any use.
+template <int N, typename KeyT>
+__device__ __forceinline__ void load_key_pairs(const DJob& J, uint32_t y, int I0, const KeyT* keys, int l,
+                                               KeyT (&cur)[N], KeyT (&prev)[N]) {
+  const KeyT* kk = keys + J.pbase;
+  const uint32_t last = J.n - 1;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint32_t p = y + (uint32_t)(I0 + i) * kWave + l;
+    const uint32_t pc = p < last ? p : last;
+    cur[i] = kk[pc];
+    prev[i] = kk[pc > 0 ? pc - 1 : 0];
+  }
+}
 template <typename KeyT>
 __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__ jobs,
                                                         const uint2* __restrict__ chunks,
@@ -602,33 +705,40 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
   // kLaneRun (empty stretches of a sparse grid) by the whole wave, one run
   // at a time, so one long run does not serialise on a single lane.
   constexpr uint64_t kLaneRun = 16;
-  for (int i = 0; i < kChunkItems; ++i) {
-    const uint32_t p0 = ch.y + (uint32_t)i * kWave;
-    if (p0 >= J.n) break;
-    const uint32_t p = p0 + (uint32_t)l;
-    uint64_t first = 0, cnt = 0;
-    if (p < J.n) {
-      const uint64_t cur = kk[p];
-      first = p == 0 ? 0 : (uint64_t)kk[p - 1] + 1;
-      cnt = cur + 1 - first;
-    }
-    const bool wide_run = cnt > kLaneRun;
-    if (!wide_run)
-      for (uint64_t e = 0; e < cnt; ++e) st[first + e] = p;
-    uint64_t bm = __ballot(wide_run);
-    while (bm) {
-      const int src = __ffsll((unsigned long long)bm) - 1;
-      bm &= bm - 1;
-      const uint64_t f = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(first >> 32), src) << 32) |
-                         __builtin_amdgcn_readlane((uint32_t)first, src);
-      const uint64_t n = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(cnt >> 32), src) << 32) |
-                         __builtin_amdgcn_readlane((uint32_t)cnt, src);
-      const uint32_t v = p0 + (uint32_t)src;
-      for (uint64_t e = (uint64_t)l; e < n; e += kWave) st[f + e] = v;
-    }
-    if (J.n - 1 - p0 < (uint32_t)kWave) {  // this step holds the job's last point
-      const uint64_t tail = (uint64_t)kk[J.n - 1] + 1;
-      for (uint64_t cell = tail + (uint64_t)l; cell <= J.cells; cell += kWave) st[cell] = J.n;
+  constexpr int kHalf = kChunkItems / 2;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    KeyT curk[kHalf], prevk[kHalf];
+    load_key_pairs<kHalf>(J, ch.y, h * kHalf, keys, l, curk, prevk);
+#pragma unroll
+    for (int ii = 0; ii < kHalf; ++ii) {
+      const uint32_t p0 = ch.y + (uint32_t)(h * kHalf + ii) * kWave;
+      if (p0 >= J.n) continue;
+      const uint32_t p = p0 + (uint32_t)l;
+      uint64_t first = 0, cnt = 0;
+      if (p < J.n) {
+        const uint64_t cur = curk[ii];
+        first = p == 0 ? 0 : (uint64_t)prevk[ii] + 1;
+        cnt = cur + 1 - first;
+      }
+      const bool wide_run = cnt > kLaneRun;
+      if (!wide_run)
+        for (uint64_t e = 0; e < cnt; ++e) st[first + e] = p;
+      uint64_t bm = __ballot(wide_run);
+      while (bm) {
+        const int src = __ffsll((unsigned long long)bm) - 1;
+        bm &= bm - 1;
+        const uint64_t f = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(first >> 32), src) << 32) |
+                           __builtin_amdgcn_readlane((uint32_t)first, src);
+        const uint64_t n = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(cnt >> 32), src) << 32) |
+                           __builtin_amdgcn_readlane((uint32_t)cnt, src);
+        const uint32_t v = p0 + (uint32_t)src;
+        for (uint64_t e = (uint64_t)l; e < n; e += kWave) st[f + e] = v;
+      }
+      if (J.n - 1 - p0 < (uint32_t)kWave) {  // this step holds the job's last point
+        const uint64_t tail = (uint64_t)kk[J.n - 1] + 1;
+        for (uint64_t cell = tail + (uint64_t)l; cell <= J.cells; cell += kWave) st[cell] = J.n;
+      }
     }
   }
 }
@@ -639,12 +749,11 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
 // have at most tq queries and span at most `group` dim-0 boxes.
 namespace {
 template <typename KeyT>
-__device__ __forceinline__ bool tile_start(const DJob& J, const KeyT* keys, const uint32_t* cstart, uint32_t p,
+__device__ __forceinline__ bool tile_start(const DJob& J, KeyT a, KeyT b, const uint32_t* cstart, uint32_t p,
                                            uint32_t tq) {
   if (p == 0 || (p % tq) == 0) return true;
   if (J.flags & kJobBrute) return false;
   // key width arithmetic (32-bit division for 32-bit keys)
-  const KeyT a = keys[J.pbase + p - 1], b = keys[J.pbase + p];
   const KeyT nb0 = (KeyT)J.nb[0];
   const KeyT ra = a / nb0, rb = b / nb0;
   if (ra != rb) return true;
@@ -671,6 +780,25 @@ __device__ __forceinline__ bool tile_start(const DJob& J, const KeyT* keys, cons
 #endif
   return true;
 }
+static_assert(kChunkItems <= 32, "tile-start flags are one 32-bit word per lane");
+// Tile-start flags of a chunk: bit i = point y + 64 i + l starts a tile.
+template <typename KeyT>
+__device__ __forceinline__ uint32_t chunk_tile_starts(const DJob& J, uint32_t y, const KeyT* keys,
+                                                      const uint32_t* cstart, uint32_t tq, int l) {
+  constexpr int kHalf = kChunkItems / 2;
+  uint32_t bits = 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    KeyT cur[kHalf], prev[kHalf];
+    load_key_pairs<kHalf>(J, y, h * kHalf, keys, l, cur, prev);
+#pragma unroll
+    for (int i = 0; i < kHalf; ++i) {
+      const uint32_t p = y + (uint32_t)(h * kHalf + i) * kWave + l;
+      if (p < J.n && tile_start(J, prev[i], cur[i], cstart, p, tq)) bits |= 1u << (h * kHalf + i);
+    }
+  }
+  return bits;
+}
 }  // namespace
 
 template <typename KeyT>
@@ -685,12 +813,10 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(const DJob* __restrict__ 
   const uint2 ch = chunks[c];
   const DJob& J = jobs[ch.x];
   const int l = lane_id();
+  const uint32_t bits = chunk_tile_starts(J, ch.y, keys, cstart, tq, l);
   uint32_t total = 0;
-  for (int i = 0; i < kChunkItems; ++i) {
-    uint32_t p = ch.y + i * kWave + l;
-    bool f = (p < J.n) && tile_start(J, keys, cstart, p, tq);
-    total += (uint32_t)__popcll(__ballot(f));
-  }
+#pragma unroll
+  for (int i = 0; i < kChunkItems; ++i) total += (uint32_t)__popcll(__ballot((bits >> i) & 1u));
   if (l == 0) cnt[c] = total;
 }
 
@@ -710,9 +836,11 @@ __global__ __launch_bounds__(kBlock) void k_tile_write(const DJob* __restrict__ 
   const int l = lane_id();
   const uint64_t lt = lanemask_lt();
   uint32_t o = offs[c];
+  const uint32_t bits = chunk_tile_starts(J, ch.y, keys, cstart, tq, l);
+#pragma unroll
   for (int i = 0; i < kChunkItems; ++i) {
-    uint32_t p = ch.y + i * kWave + l;
-    bool f = (p < J.n) && tile_start(J, keys, cstart, p, tq);
+    const uint32_t p = ch.y + i * kWave + l;
+    const bool f = (bits >> i) & 1u;
     uint64_t b = __ballot(f);
     if (f) {
       DTile t;
