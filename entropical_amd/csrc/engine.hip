@@ -56,7 +56,8 @@ Engine::~Engine() {
   DevBuf* bufs[] = {&d_jobs,  &d_chunks, &d_chunk_first, &d_mm,     &d_flags,        &d_keys[0],
                     &d_keys[1], &d_idx[0], &d_idx[1],     &d_hist,   &d_scan_scratch, &d_scoord,
                     &d_tile_cnt, &d_tiles, &d_counter, &d_cellstart, &d_bjobs, &d_bchunks,
-                    &d_bchunk_first, &d_bbase, &d_bkeys[0], &d_bkeys[1], &d_bidx[0], &d_bidx[1]};
+                    &d_bchunk_first, &d_bbase, &d_bkeys[0], &d_bkeys[1], &d_bidx[0], &d_bidx[1],
+                    &d_wjobs, &d_wchunks};
   for (GraphCache* c : {&pre_graphs_, &dens_graphs_})
     for (auto& e : c->entries) hipGraphExecDestroy(e.second);
   for (DevBuf* b : bufs)
@@ -164,24 +165,57 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   const uint32_t J = (uint32_t)jobs.size();
   const uint32_t NC = (uint32_t)h_chunks.size();
   const uint64_t P = total_points;
+  // Min/max per distinct 1-D window (column offset, length) rather than per
+  // job and dim: the jobs of a batch share columns (the MI matrix's 4160
+  // job dims read 64 windows).
+  {
+    std::map<std::pair<uint64_t, uint32_t>, uint32_t> wid;
+    h_wjobs.clear();
+    h_wchunks.clear();
+    h_jw.assign(3 * (size_t)J, 0u);
+    for (uint32_t j = 0; j < J; ++j)
+      for (uint32_t e = 0; e < h_jobs[j].d; ++e) {
+        const auto key = std::make_pair(h_jobs[j].off[e], h_jobs[j].n);
+        auto ins = wid.emplace(key, (uint32_t)h_wjobs.size());
+        if (ins.second) {
+          DJob W;
+          std::memset(&W, 0, sizeof(W));
+          W.d = 1;
+          W.n = key.second;
+          W.off[0] = key.first;
+          for (uint32_t st = 0; st < W.n; st += kChunk)
+            h_wchunks.push_back(make_uint2((uint32_t)h_wjobs.size(), st));
+          h_wjobs.push_back(W);
+        }
+        h_jw[3 * (size_t)j + e] = ins.first->second;
+      }
+  }
+  const uint32_t NW = (uint32_t)h_wjobs.size();
+  const uint32_t NWC = (uint32_t)h_wchunks.size();
   ENT_TRY(ensure(d_jobs, sizeof(DJob) * J));
   ENT_TRY(ensure(d_chunks, sizeof(uint2) * NC));
   ENT_TRY(ensure(d_chunk_first, sizeof(uint32_t) * (J + 1)));
-  ENT_TRY(ensure(d_mm, sizeof(uint32_t) * 6 * J));
+  ENT_TRY(ensure(d_wjobs, sizeof(DJob) * NW));
+  ENT_TRY(ensure(d_wchunks, sizeof(uint2) * NWC));
+  ENT_TRY(ensure(d_mm, sizeof(uint32_t) * 6 * NW));
   ENT_TRY(ensure(d_flags, 64));
-  // host staging: jobs + chunks + chunk_first + minmax readback
+  // host staging: jobs + chunks + chunk_first + windows + minmax readback
   size_t stage_bytes = sizeof(DJob) * J + sizeof(uint2) * NC + sizeof(uint32_t) * (J + 1) +
-                       sizeof(uint32_t) * 6 * J + 64;
+                       8 + sizeof(DJob) * NW + sizeof(uint2) * NWC + sizeof(uint32_t) * 6 * NW + 64;
   ENT_TRY(ensure_host(stage_bytes));
   char* hs = (char*)host_stage_;
   DJob* hs_jobs = (DJob*)hs;
   uint2* hs_chunks = (uint2*)(hs_jobs + J);
   uint32_t* hs_cf = (uint32_t*)(hs_chunks + NC);
-  uint32_t* hs_mm = hs_cf + (J + 1);
-  uint32_t* hs_flags = hs_mm + 6 * J;
+  DJob* hs_wjobs = (DJob*)(((uintptr_t)(hs_cf + (J + 1)) + 7) & ~(uintptr_t)7);
+  uint2* hs_wchunks = (uint2*)(hs_wjobs + NW);
+  uint32_t* hs_mm = (uint32_t*)(hs_wchunks + NWC);
+  uint32_t* hs_flags = hs_mm + 6 * NW;
   std::memcpy(hs_jobs, h_jobs.data(), sizeof(DJob) * J);
   std::memcpy(hs_chunks, h_chunks.data(), sizeof(uint2) * NC);
   std::memcpy(hs_cf, h_chunk_first.data(), sizeof(uint32_t) * (J + 1));
+  std::memcpy(hs_wjobs, h_wjobs.data(), sizeof(DJob) * NW);
+  std::memcpy(hs_wchunks, h_wchunks.data(), sizeof(uint2) * NWC);
 
   ENT_HIP(hipEventRecord(ev_[0], stream_));
   ENT_HIP(hipMemcpyAsync(d_jobs.p, hs_jobs, sizeof(DJob) * J, hipMemcpyHostToDevice, stream_));
@@ -190,11 +224,14 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   const DJob* dj = (const DJob*)d_jobs.p;
   const uint2* dch = (const uint2*)d_chunks.p;
   const uint32_t* dcf = (const uint32_t*)d_chunk_first.p;
-  launch_init_minmax(stream_, (uint32_t*)d_mm.p, J);
+  ENT_HIP(hipMemcpyAsync(d_wjobs.p, hs_wjobs, sizeof(DJob) * NW, hipMemcpyHostToDevice, stream_));
+  ENT_HIP(hipMemcpyAsync(d_wchunks.p, hs_wchunks, sizeof(uint2) * NWC, hipMemcpyHostToDevice, stream_));
+  launch_init_minmax(stream_, (uint32_t*)d_mm.p, NW);
   launch_fill_u32(stream_, (uint32_t*)d_flags.p, 0u, 16);
-  launch_minmax(stream_, dj, dch, NC, X, (uint32_t*)d_mm.p, (uint32_t*)d_flags.p);
+  launch_minmax(stream_, (const DJob*)d_wjobs.p, (const uint2*)d_wchunks.p, NWC, X, (uint32_t*)d_mm.p,
+                (uint32_t*)d_flags.p);
   ENT_HIP(hipGetLastError());
-  ENT_HIP(hipMemcpyAsync(hs_mm, d_mm.p, sizeof(uint32_t) * 6 * J, hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipMemcpyAsync(hs_mm, d_mm.p, sizeof(uint32_t) * 6 * NW, hipMemcpyDeviceToHost, stream_));
   ENT_HIP(hipMemcpyAsync(hs_flags, d_flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
   ENT_HIP(hipStreamSynchronize(stream_));
   if (hs_flags[0] != 0) {
@@ -208,8 +245,9 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     DJob& D = h_jobs[j];
     uint64_t cells = 1;
     for (uint32_t e = 0; e < D.d; ++e) {
-      float mn = o2f(hs_mm[j * 6 + e * 2]);
-      float mx = o2f(hs_mm[j * 6 + e * 2 + 1]);
+      const uint32_t w = h_jw[3 * (size_t)j + e];
+      float mn = o2f(hs_mm[w * 6]);
+      float mx = o2f(hs_mm[w * 6 + 1]);
       float m = (float)(e == 0 ? tuning.m0 : tuning.mr);
       volatile float bs = D.h[e] / m;
       volatile float range = mx - mn;

@@ -139,6 +139,8 @@ class Engine {
       d_scan_scratch, d_scoord, d_tile_cnt, d_tiles, d_counter, d_cellstart;
   // two-stage sort: base (shared dim-0 sort) tables and buffers
   DevBuf d_bjobs, d_bchunks, d_bchunk_first, d_bbase, d_bkeys[2], d_bidx[2];
+  // min/max windows: the distinct (column offset, length) of all job dims
+  DevBuf d_wjobs, d_wchunks;
   // host copies
   std::vector<DJob> h_jobs;
   std::vector<uint2> h_chunks;
@@ -147,6 +149,9 @@ class Engine {
   std::vector<uint2> h_bchunks;
   std::vector<uint32_t> h_bchunk_first;
   std::vector<uint64_t> h_bbase;  // per job: first slot of its base
+  std::vector<DJob> h_wjobs;      // one 1-D job per min/max window
+  std::vector<uint2> h_wchunks;
+  std::vector<uint32_t> h_jw;     // per job and dim (3 per job): its window
   uint64_t total_points = 0;
   int sorted_buf = 0;  // which of d_keys/d_idx holds the sorted result
   bool keys_wide = false;  // cell keys stored as uint64 (else uint32)
