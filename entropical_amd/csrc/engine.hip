@@ -339,6 +339,10 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   bool staged = true;
   if (const char* m = std::getenv("ENT_SCATTER"))
     if (!std::strcmp(m, "direct")) staged = false;
+  // packing of small tiles (on unless ENT_TILE_PACK=0)
+  bool pack = true;
+  if (const char* m = std::getenv("ENT_TILE_PACK"))
+    if (!std::strcmp(m, "0")) pack = false;
   const uint32_t NB = (uint32_t)h_bjobs.size();
   const uint32_t NCB = (uint32_t)h_bchunks.size();
   const int npass0 = shared ? passes_for(max_b0) : 0;
@@ -445,12 +449,11 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     }
     if (!with_tiles) return;
     if (table_total) launch_cell_starts(stream_, wide, dj, dch, NC, keys[cur], (uint32_t*)d_cellstart.p);
-    const uint32_t* cst = (const uint32_t*)d_cellstart.p;
-    launch_tile_count(stream_, wide, dj, dch, NC, keys[cur], cst, tq, tcnt);
+    launch_tile_count(stream_, wide, dj, dch, NC, keys[cur], tq, tcnt);
     launch_exclusive_scan(stream_, tcnt, tcnt, NC, scratch);
-    launch_tile_write(stream_, wide, dj, dch, NC, keys[cur], cst, tq, tcnt, (DTile*)d_tiles.p);
+    launch_tile_write(stream_, wide, dj, dch, NC, keys[cur], tq, tcnt, (DTile*)d_tiles.p);
     launch_tile_ends(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
-    launch_tile_pack(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
+    if (pack) launch_tile_pack(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
   };
   if (stop == kStopAfterSort) {
     issue_pre(false);
@@ -460,7 +463,7 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   PlanKey key = plan_key(X, dens, J, NC, P, npass, false);
   key.NCB = shared ? NCB : 0;
   key.PB = shared ? PB : 0;
-  key.variant = (shared ? 1 : 0) | (staged ? 2 : 0) | (npass0 << 2);
+  key.variant = (shared ? 1 : 0) | (staged ? 2 : 0) | (pack ? 4 : 0) | (npass0 << 3);
   ENT_TRY(run_captured(pre_graphs_, key, [&] { issue_pre(true); }));
   return launch_density_common(false, X, dens, P);
 }

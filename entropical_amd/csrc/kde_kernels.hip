@@ -13,16 +13,6 @@
 #ifndef ENT_DENSITY_MIN_WAVES
 #define ENT_DENSITY_MIN_WAVES 1
 #endif
-// Packing of small tiles into one wave (0: every tile on its own).
-#ifndef ENT_TILE_PACK
-#define ENT_TILE_PACK 1
-#endif
-// Tile boundaries inside a row are dropped between dim-0 groups whose common
-// aligned block of at most 2^ENT_MERGE_LEVELS groups holds at most one tile
-// of points (0: every group starts a tile).
-#ifndef ENT_MERGE_LEVELS
-#define ENT_MERGE_LEVELS 0
-#endif
 
 namespace ent {
 namespace {
@@ -749,42 +739,20 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
 // have at most tq queries and span at most `group` dim-0 boxes.
 namespace {
 template <typename KeyT>
-__device__ __forceinline__ bool tile_start(const DJob& J, KeyT a, KeyT b, const uint32_t* cstart, uint32_t p,
-                                           uint32_t tq) {
+__device__ __forceinline__ bool tile_start(const DJob& J, KeyT a, KeyT b, uint32_t p, uint32_t tq) {
   if (p == 0 || (p % tq) == 0) return true;
   if (J.flags & kJobBrute) return false;
   // key width arithmetic (32-bit division for 32-bit keys)
   const KeyT nb0 = (KeyT)J.nb[0];
   const KeyT ra = a / nb0, rb = b / nb0;
   if (ra != rb) return true;
-  const uint32_t ga = (uint32_t)(a - ra * nb0) / J.group, gb = (uint32_t)(b - rb * nb0) / J.group;
-  if (ga == gb) return false;
-#if ENT_MERGE_LEVELS > 0
-  // Sparse stretches of a row: groups merge into one tile when the smallest
-  // aligned block of 2^L groups holding both sides of the step (L <=
-  // ENT_MERGE_LEVELS) has at most tq points, counted from the cell-start
-  // table.  Every tile between two kept steps then lies inside one such
-  // block (a run crossing a block edge of level L steps over a pair whose
-  // common block is above L).
-  if (J.cbase != kNoTable) {
-    const int L = 32 - __builtin_clz(ga ^ gb);
-    if (L <= ENT_MERGE_LEVELS) {
-      const uint64_t row0 = (uint64_t)rb * J.nb[0];
-      const uint64_t g_lo = (uint64_t)((gb >> L) << L) * J.group;
-      const uint64_t g_end = g_lo + ((uint64_t)J.group << L);
-      const uint64_t g_hi = g_end < J.nb[0] ? g_end : J.nb[0];
-      const uint32_t* st = cstart + J.cbase;
-      if (st[row0 + g_hi] - st[row0 + g_lo] <= tq) return false;
-    }
-  }
-#endif
-  return true;
+  return (uint32_t)(a - ra * nb0) / J.group != (uint32_t)(b - rb * nb0) / J.group;
 }
 static_assert(kChunkItems <= 32, "tile-start flags are one 32-bit word per lane");
 // Tile-start flags of a chunk: bit i = point y + 64 i + l starts a tile.
 template <typename KeyT>
 __device__ __forceinline__ uint32_t chunk_tile_starts(const DJob& J, uint32_t y, const KeyT* keys,
-                                                      const uint32_t* cstart, uint32_t tq, int l) {
+                                                      uint32_t tq, int l) {
   constexpr int kHalf = kChunkItems / 2;
   uint32_t bits = 0;
 #pragma unroll
@@ -794,7 +762,7 @@ __device__ __forceinline__ uint32_t chunk_tile_starts(const DJob& J, uint32_t y,
 #pragma unroll
     for (int i = 0; i < kHalf; ++i) {
       const uint32_t p = y + (uint32_t)(h * kHalf + i) * kWave + l;
-      if (p < J.n && tile_start(J, prev[i], cur[i], cstart, p, tq)) bits |= 1u << (h * kHalf + i);
+      if (p < J.n && tile_start(J, prev[i], cur[i], p, tq)) bits |= 1u << (h * kHalf + i);
     }
   }
   return bits;
@@ -806,14 +774,13 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(const DJob* __restrict__ 
                                                        const uint2* __restrict__ chunks,
                                                        uint32_t n_chunks,
                                                        const KeyT* __restrict__ keys,
-                                                       const uint32_t* __restrict__ cstart,
                                                        uint32_t tq, uint32_t* __restrict__ cnt) {
   const uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (c >= n_chunks) return;
   const uint2 ch = chunks[c];
   const DJob& J = jobs[ch.x];
   const int l = lane_id();
-  const uint32_t bits = chunk_tile_starts(J, ch.y, keys, cstart, tq, l);
+  const uint32_t bits = chunk_tile_starts(J, ch.y, keys, tq, l);
   uint32_t total = 0;
 #pragma unroll
   for (int i = 0; i < kChunkItems; ++i) total += (uint32_t)__popcll(__ballot((bits >> i) & 1u));
@@ -825,7 +792,6 @@ __global__ __launch_bounds__(kBlock) void k_tile_write(const DJob* __restrict__ 
                                                        const uint2* __restrict__ chunks,
                                                        uint32_t n_chunks,
                                                        const KeyT* __restrict__ keys,
-                                                       const uint32_t* __restrict__ cstart,
                                                        uint32_t tq,
                                                        const uint32_t* __restrict__ offs,
                                                        DTile* __restrict__ tiles) {
@@ -836,7 +802,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_write(const DJob* __restrict__ 
   const int l = lane_id();
   const uint64_t lt = lanemask_lt();
   uint32_t o = offs[c];
-  const uint32_t bits = chunk_tile_starts(J, ch.y, keys, cstart, tq, l);
+  const uint32_t bits = chunk_tile_starts(J, ch.y, keys, tq, l);
 #pragma unroll
   for (int i = 0; i < kChunkItems; ++i) {
     const uint32_t p = ch.y + i * kWave + l;
@@ -1556,24 +1522,23 @@ void launch_gather_sorted(hipStream_t s, const DJob* jobs, const uint2* chunks, 
 }
 
 void launch_tile_count(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const void* keys, const uint32_t* cstart, uint32_t tq, uint32_t* cnt) {
+                       const void* keys, uint32_t tq, uint32_t* cnt) {
   if (wide)
     hipLaunchKernelGGL(k_tile_count<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint64_t*)keys, cstart, tq, cnt);
+                       n_chunks, (const uint64_t*)keys, tq, cnt);
   else
     hipLaunchKernelGGL(k_tile_count<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint32_t*)keys, cstart, tq, cnt);
+                       n_chunks, (const uint32_t*)keys, tq, cnt);
 }
 
 void launch_tile_write(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const void* keys, const uint32_t* cstart, uint32_t tq, const uint32_t* offs,
-                       DTile* tiles) {
+                       const void* keys, uint32_t tq, const uint32_t* offs, DTile* tiles) {
   if (wide)
     hipLaunchKernelGGL(k_tile_write<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint64_t*)keys, cstart, tq, offs, tiles);
+                       n_chunks, (const uint64_t*)keys, tq, offs, tiles);
   else
     hipLaunchKernelGGL(k_tile_write<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint32_t*)keys, cstart, tq, offs, tiles);
+                       n_chunks, (const uint32_t*)keys, tq, offs, tiles);
 }
 
 void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
@@ -1585,14 +1550,10 @@ void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint3
 }
 void launch_tile_pack(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
                       uint64_t max_tiles) {
-#if ENT_TILE_PACK
   uint64_t blocks = (max_tiles / 4 + kBlock) / kBlock;
   if (blocks > 4096) blocks = 4096;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_tile_pack, dim3((unsigned)blocks), dim3(kBlock), 0, s, jobs, tiles, total);
-#else
-  (void)s; (void)jobs; (void)tiles; (void)total; (void)max_tiles;
-#endif
 }
 
 // Kernel variants: RAW (plain path, keys unused) or cell lists with 32- or

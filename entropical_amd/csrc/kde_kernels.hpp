@@ -34,13 +34,11 @@ void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uin
                            uint32_t* scratch);
 void launch_gather_sorted(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
                           const float* X, const uint32_t* idx, float* scoord);
-// Tile boundaries: every tq points, at row changes and between dim-0 groups
-// (cstart: the cell-start tables, consulted to merge sparse groups).
+// Tile boundaries: every tq points, at row changes and between dim-0 groups.
 void launch_tile_count(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const void* keys, const uint32_t* cstart, uint32_t tq, uint32_t* cnt);
+                       const void* keys, uint32_t tq, uint32_t* cnt);
 void launch_tile_write(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const void* keys, const uint32_t* cstart, uint32_t tq, const uint32_t* offs,
-                       DTile* tiles);
+                       const void* keys, uint32_t tq, const uint32_t* offs, DTile* tiles);
 void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
                       uint64_t max_tiles);
 // Packs runs of small tiles of one job for side-by-side processing (in place).

@@ -196,6 +196,29 @@ def test_cell_sort_bit_exact(ctx, orc, monkeypatch, sort_mode, scatter):
         np.testing.assert_array_equal(keys, key[order])
 
 
+@pytest.mark.parametrize("d", [2, 3])
+def test_packed_small_tiles(ctx, orc, monkeypatch, d):
+    """Sparse tails give tiles of a few queries, which run packed two or four
+    to a wave (16 or 32 lanes per tile).  Densities with packing and without
+    it (ENT_TILE_PACK=0) both match FP64, and differ from each other in the
+    last bits (each sub-tile has its own origin), so the packed path ran."""
+    from entropical_amd import combined_densities
+
+    r = np.random.default_rng(31 + d)
+    n = 6000
+    x = r.standard_t(3, size=(3, n)).astype(np.float32)
+    cols = [0, 1, 2][:d]
+    h = [0.1, 0.12, 0.15][:d]
+    ref = orc.density_fp64(x, n, cols, h)
+    monkeypatch.setenv("ENT_TILE_PACK", "0")
+    single = combined_densities(x, n, cols, h, ctx=ctx)
+    monkeypatch.setenv("ENT_TILE_PACK", "1")
+    packed = combined_densities(x, n, cols, h, ctx=ctx)
+    assert rel_err(single, ref) <= RTOL
+    assert rel_err(packed, ref) <= RTOL
+    assert not np.array_equal(single, packed)
+
+
 # ---------------------------------------------------------- epa convolution
 def test_epa_convolution_matches_fp64(ctx, orc):
     from entropical_amd import epa_convolution
