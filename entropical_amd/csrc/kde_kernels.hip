@@ -835,47 +835,52 @@ __global__ void k_tile_ends(const DJob* __restrict__ jobs, DTile* __restrict__ t
   }
 }
 
-// Packed tiles: each aligned quad of tiles 4q .. 4q+3 from one job (d >= 2)
-// with at most 16 queries each becomes one packed tile of four sub-tiles;
-// otherwise each aligned pair from one job with at most 32 queries each
-// becomes a packed tile of two.  The first tile of a pack takes the whole
-// span and the sub-tile offsets; the others are marked kTileSkip.  One
-// thread per quad rewrites its four tiles in place.
+// Packed tiles: each thread walks a run of kPackRun consecutive tiles and
+// packs greedily: four tiles of one job (d >= 2) with at most 16 queries
+// each become one packed tile of four sub-tiles, else two with at most 32
+// each become one of two.  The first tile of a pack takes the whole span and
+// the sub-tile offsets; the others are marked kTileSkip.  Packs do not cross
+// run boundaries, so the threads rewrite disjoint tiles in place.
+constexpr uint32_t kPackRun = 32;
 __global__ void k_tile_pack(const DJob* __restrict__ jobs, DTile* __restrict__ tiles,
                             const uint32_t* __restrict__ total) {
   const uint32_t T = *total;
-  for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; 4 * q < T;
-       q += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t t0 = (uint32_t)(4 * q);
-    const uint32_t n = T - t0 < 4u ? T - t0 : 4u;
-    DTile t[4];
-    for (uint32_t i = 0; i < n; ++i) t[i] = tiles[t0 + i];
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r * kPackRun < T;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t t0 = (uint32_t)(r * kPackRun);
+    const uint32_t t1 = T - t0 < kPackRun ? T : t0 + kPackRun;
+    // tiles [a, a + cnt) of one job (d >= 2) with at most lim queries each
     auto fits = [&](uint32_t a, uint32_t cnt, uint32_t lim) {
-      if (a + cnt > n) return false;
-      const DJob& J = jobs[t[a].job];
+      if (a + cnt > t1) return false;
+      const DTile first = tiles[a];
+      const DJob& J = jobs[first.job];
       if (J.d < 2 || (J.flags & kJobBrute)) return false;
-      for (uint32_t i = a; i < a + cnt; ++i)
-        if (t[i].job != t[a].job || t[i].p1 - t[i].p0 > lim) return false;
+      for (uint32_t i = a; i < a + cnt; ++i) {
+        const DTile ti = tiles[i];
+        if (ti.job != first.job || ti.p1 - ti.p0 > lim) return false;
+      }
       return true;
     };
     auto pack = [&](uint32_t a, uint32_t cnt) {
-      DTile h = t[a];
+      DTile h = tiles[a];
       uint32_t code = cnt - 1;
-      for (uint32_t i = 1; i < cnt; ++i) code |= (t[a + i].p0 - h.p0) << (8 * i);
-      h.p1 = t[a + cnt - 1].p1;
+      for (uint32_t i = 1; i < cnt; ++i) code |= (tiles[a + i].p0 - h.p0) << (8 * i);
+      h.p1 = tiles[a + cnt - 1].p1;
       h.pack = code;
-      tiles[t0 + a] = h;
-      for (uint32_t i = 1; i < cnt; ++i) {
-        DTile sk = t[a + i];
-        sk.pack = kTileSkip;
-        tiles[t0 + a + i] = sk;
-      }
+      for (uint32_t i = 1; i < cnt; ++i) tiles[a + i].pack = kTileSkip;
+      tiles[a] = h;
     };
-    if (fits(0, 4, 16)) {
-      pack(0, 4);
-    } else {
-      if (fits(0, 2, 32)) pack(0, 2);
-      if (fits(2, 2, 32)) pack(2, 2);
+    uint32_t a = t0;
+    while (a < t1) {
+      if (fits(a, 4, 16)) {
+        pack(a, 4);
+        a += 4;
+      } else if (fits(a, 2, 32)) {
+        pack(a, 2);
+        a += 2;
+      } else {
+        ++a;
+      }
     }
   }
 }
@@ -1550,7 +1555,7 @@ void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint3
 }
 void launch_tile_pack(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
                       uint64_t max_tiles) {
-  uint64_t blocks = (max_tiles / 4 + kBlock) / kBlock;
+  uint64_t blocks = (max_tiles / kPackRun + kBlock) / kBlock;
   if (blocks > 4096) blocks = 4096;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_tile_pack, dim3((unsigned)blocks), dim3(kBlock), 0, s, jobs, tiles, total);
