@@ -20,15 +20,17 @@ __device__ __forceinline__ double inv0(double x) { return x == 0.0 ? x : 1.0 / x
 __device__ __forceinline__ double term(const RedItem& it, uint64_t k, const double* __restrict__ D) {
   const double a = D[it.off[0] + k];
   switch (it.op) {
-    case kRedMI: {  // mi.cpp:53-56: p_ij * log(p_ij / p_i / p_j)
-      const double r = a / D[it.off[1] + k] / D[it.off[2] + k];
+    case kRedMI: {  // mi.cpp:53-56: p_ij * log(p_ij / p_i / p_j), one division
+      const double r = a / (D[it.off[1] + k] * D[it.off[2] + k]);
       return r == 0.0 ? 0.0 : a * log(r) * it.scale;
     }
     case kRedLogMean:  // sum of ln p (entropy = -mean)
       return log(a);
     case kRedTE: {  // transs.cpp:77-100: p3*pref*log(p3*p1*inv(p2)*inv(ptau))
       if (!(a > 0.0)) return 0.0;
-      const double r = a * D[it.off[1] + k] * inv0(D[it.off[2] + k]) * inv0(D[it.off[3] + k]);
+      // inv(p2) * inv(ptau) as one division (0 when either is 0, as inv does)
+      const double den = D[it.off[2] + k] * D[it.off[3] + k];
+      const double r = den == 0.0 ? 0.0 : a * D[it.off[1] + k] / den;
       if (r == 0.0) return 0.0;
       const double te = a * it.pref * log(r) * it.scale;
       return (isfinite(te) && a <= 1.0) ? te : 0.0;
