@@ -57,7 +57,7 @@ Engine::~Engine() {
                     &d_keys[1], &d_idx[0], &d_idx[1],     &d_hist,   &d_scan_scratch, &d_scoord,
                     &d_tile_cnt, &d_tiles, &d_counter, &d_cellstart, &d_bjobs, &d_bchunks,
                     &d_bchunk_first, &d_bbase, &d_bkeys[0], &d_bkeys[1], &d_bidx[0], &d_bidx[1],
-                    &d_wjobs, &d_wchunks};
+                    &d_wjobs, &d_wchunks, &d_tchunks};
   for (GraphCache* c : {&pre_graphs_, &dens_graphs_})
     for (auto& e : c->entries) hipGraphExecDestroy(e.second);
   for (DevBuf* b : bufs)
@@ -350,16 +350,57 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   last.radix_passes = (uint32_t)(npass0 + npass);
   last.sort_bases = shared ? NB : 0;
 
+  // Tile order: the tile list follows this chunk order, jobs with the most
+  // candidates per query (n times the fraction of the range within +-h per
+  // dim) first, so the longest tiles are claimed early and the kernel's
+  // tail holds short ones.  Uploaded only when it changes.
+  bool upload_tchunks = false;
+  {
+    std::vector<double> est(J);
+    for (uint32_t j = 0; j < J; ++j) {
+      const DJob& D = h_jobs[j];
+      double c = (double)D.n;
+      for (uint32_t e = 0; e < D.d; ++e) {
+        const double range = (double)D.nb[e] * (double)D.bs[e];
+        c *= range > 0.0 ? std::min(1.0, 2.0 * (double)D.h[e] / range) : 1.0;
+      }
+      est[j] = c;
+    }
+    std::vector<uint32_t> ord(J);
+    for (uint32_t j = 0; j < J; ++j) ord[j] = j;
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return est[a] > est[b]; });
+    const size_t tchunks_cap = d_tchunks.cap;
+    ENT_TRY(ensure(d_tchunks, sizeof(uint2) * NC));
+    // a reallocated buffer holds nothing; an unchanged order (same jobs,
+    // chunks and order as the last upload) needs no upload
+    std::vector<uint32_t> sizes(J);
+    for (uint32_t j = 0; j < J; ++j) sizes[j] = h_jobs[j].n;  // the chunk list follows from these
+    upload_tchunks = d_tchunks.cap != tchunks_cap || ord != h_tord || sizes != h_tsizes;
+    if (upload_tchunks) {
+      h_tchunks.clear();
+      h_tchunks.reserve(NC);
+      for (uint32_t j : ord)
+        for (uint32_t c = h_chunk_first[j]; c < h_chunk_first[j + 1]; ++c) h_tchunks.push_back(h_chunks[c]);
+      h_tord = ord;
+      h_tsizes = sizes;
+    }
+  }
+
   // upload the final job table (and the base tables)
   const size_t base_bytes = shared ? sizeof(DJob) * NB + sizeof(uint2) * NCB + sizeof(uint32_t) * (NB + 1) +
                                          sizeof(uint64_t) * J
                                    : 0;
-  ENT_TRY(ensure_host(sizeof(DJob) * J + base_bytes + 64));
+  ENT_TRY(ensure_host(sizeof(DJob) * J + base_bytes + (upload_tchunks ? sizeof(uint2) * NC : 0) + 64));
   {
     char* p = (char*)host_stage_;
     std::memcpy(p, h_jobs.data(), sizeof(DJob) * J);
     ENT_HIP(hipMemcpyAsync(d_jobs.p, p, sizeof(DJob) * J, hipMemcpyHostToDevice, stream_));
     p += sizeof(DJob) * J;
+    if (upload_tchunks) {
+      std::memcpy(p, h_tchunks.data(), sizeof(uint2) * NC);
+      ENT_HIP(hipMemcpyAsync(d_tchunks.p, p, sizeof(uint2) * NC, hipMemcpyHostToDevice, stream_));
+      p += sizeof(uint2) * NC;
+    }
     if (shared) {
       ENT_TRY(ensure(d_bjobs, sizeof(DJob) * NB));
       ENT_TRY(ensure(d_bchunks, sizeof(uint2) * NCB));
@@ -449,9 +490,10 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     }
     if (!with_tiles) return;
     if (table_total) launch_cell_starts(stream_, wide, dj, dch, NC, keys[cur], (uint32_t*)d_cellstart.p);
-    launch_tile_count(stream_, wide, dj, dch, NC, keys[cur], tq, tcnt);
+    const uint2* tch = (const uint2*)d_tchunks.p;
+    launch_tile_count(stream_, wide, dj, tch, NC, keys[cur], tq, tcnt);
     launch_exclusive_scan(stream_, tcnt, tcnt, NC, scratch);
-    launch_tile_write(stream_, wide, dj, dch, NC, keys[cur], tq, tcnt, (DTile*)d_tiles.p);
+    launch_tile_write(stream_, wide, dj, tch, NC, keys[cur], tq, tcnt, (DTile*)d_tiles.p);
     launch_tile_ends(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
     if (pack) launch_tile_pack(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
   };

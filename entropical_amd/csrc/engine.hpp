@@ -141,6 +141,8 @@ class Engine {
   DevBuf d_bjobs, d_bchunks, d_bchunk_first, d_bbase, d_bkeys[2], d_bidx[2];
   // min/max windows: the distinct (column offset, length) of all job dims
   DevBuf d_wjobs, d_wchunks;
+  // chunk list in tile order (jobs with the most candidates per query first)
+  DevBuf d_tchunks;
   // host copies
   std::vector<DJob> h_jobs;
   std::vector<uint2> h_chunks;
@@ -152,6 +154,9 @@ class Engine {
   std::vector<DJob> h_wjobs;      // one 1-D job per min/max window
   std::vector<uint2> h_wchunks;
   std::vector<uint32_t> h_jw;     // per job and dim (3 per job): its window
+  std::vector<uint2> h_tchunks;      // chunk list in tile order (last upload)
+  std::vector<uint32_t> h_tord;      // job order of the last upload
+  std::vector<uint32_t> h_tsizes;    // job sizes of the last upload
   uint64_t total_points = 0;
   int sorted_buf = 0;  // which of d_keys/d_idx holds the sorted result
   bool keys_wide = false;  // cell keys stored as uint64 (else uint32)
