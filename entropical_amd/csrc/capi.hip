@@ -129,12 +129,13 @@ int ent_ctx_create(int device, ent_ctx** out) {
 int ent_ctx_destroy(ent_ctx* ctx) {
   if (!ctx) return ENT_OK;
   if (ctx->eng) {
-    hipSetDevice(ctx->eng->device());
-    hipStreamSynchronize(ctx->eng->stream());
-    if (ctx->upload.p) hipFree(ctx->upload.p);
-    if (ctx->out.p) hipFree(ctx->out.p);
-    if (ctx->aux.p) hipFree(ctx->aux.p);
-    if (ctx->red.p) hipFree(ctx->red.p);
+    // teardown: errors here are not actionable
+    (void)hipSetDevice(ctx->eng->device());
+    (void)hipStreamSynchronize(ctx->eng->stream());
+    if (ctx->upload.p) (void)hipFree(ctx->upload.p);
+    if (ctx->out.p) (void)hipFree(ctx->out.p);
+    if (ctx->aux.p) (void)hipFree(ctx->aux.p);
+    if (ctx->red.p) (void)hipFree(ctx->red.p);
   }
   delete ctx->eng;
   delete ctx;

@@ -46,26 +46,28 @@ Engine::Engine(int device) : device_(device) {
   if (hipSetDevice(device) != hipSuccess) return;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) num_cus_ = prop.multiProcessorCount;
-  hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
-  for (auto& e : ev_) hipEventCreate(&e);
+  // failures leave stream_/events null; the first use reports the error
+  (void)hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
+  for (auto& e : ev_) (void)hipEventCreate(&e);
 }
 
 Engine::~Engine() {
-  hipSetDevice(device_);
-  if (stream_) hipStreamSynchronize(stream_);
+  // teardown: errors here have no one to report to
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
   DevBuf* bufs[] = {&d_jobs,  &d_chunks, &d_chunk_first, &d_mm,     &d_flags,        &d_keys[0],
                     &d_keys[1], &d_idx[0], &d_idx[1],     &d_hist,   &d_scan_scratch, &d_scoord,
                     &d_tile_cnt, &d_tiles, &d_counter, &d_cellstart, &d_bjobs, &d_bchunks,
                     &d_bchunk_first, &d_bbase, &d_bkeys[0], &d_bkeys[1], &d_bidx[0], &d_bidx[1],
                     &d_wjobs, &d_wchunks, &d_tchunks};
   for (GraphCache* c : {&pre_graphs_, &dens_graphs_})
-    for (auto& e : c->entries) hipGraphExecDestroy(e.second);
+    for (auto& e : c->entries) (void)hipGraphExecDestroy(e.second);
   for (DevBuf* b : bufs)
-    if (b->p) hipFree(b->p);
+    if (b->p) (void)hipFree(b->p);
   for (auto& e : ev_)
-    if (e) hipEventDestroy(e);
-  if (host_stage_) hipHostFree(host_stage_);
-  if (stream_) hipStreamDestroy(stream_);
+    if (e) (void)hipEventDestroy(e);
+  if (host_stage_) (void)hipHostFree(host_stage_);
+  if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 int Engine::ensure(DevBuf& b, size_t bytes) {
@@ -548,7 +550,7 @@ int Engine::run_captured(GraphCache& cache, const PlanKey& key, F&& issue) {
   if (cache.entries.size() >= GraphCache::kMax) {
     // the stream may still run the oldest graph; wait before destroying it
     ENT_HIP(hipStreamSynchronize(stream_));
-    hipGraphExecDestroy(cache.entries.front().second);
+    (void)hipGraphExecDestroy(cache.entries.front().second);
     cache.entries.erase(cache.entries.begin());
   }
   hipGraphExec_t exec = nullptr;
@@ -558,12 +560,12 @@ int Engine::run_captured(GraphCache& cache, const PlanKey& key, F&& issue) {
   hipError_t le = hipGetLastError();
   hipError_t ce = hipStreamEndCapture(stream_, &g);
   if (le != hipSuccess || ce != hipSuccess) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     set_error(std::string("graph capture failed: ") + hipGetErrorString(le != hipSuccess ? le : ce));
     return kErrHip;
   }
   hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   if (ie != hipSuccess) {
     set_error(std::string("graph instantiation failed: ") + hipGetErrorString(ie));
     return kErrHip;

@@ -184,11 +184,12 @@ struct Rng {
 
 int ensure_batch(ent_mlp* M, size_t B) {
   if (B <= M->batch_cap) return ENT_OK;
-  hipFree(M->act);
-  hipFree(M->dact);
-  hipFree(M->xb);
-  hipFree(M->tb);
-  hipFree(M->idx);
+  // the old buffers are dropped before the larger ones are allocated
+  (void)hipFree(M->act);
+  (void)hipFree(M->dact);
+  (void)hipFree(M->xb);
+  (void)hipFree(M->tb);
+  (void)hipFree(M->idx);
   size_t per = 0;
   for (int l = 0; l <= M->n_layers; ++l) per += (size_t)M->dims[l];
   MLP_HIP(hipMalloc(&M->act, sizeof(float) * B * per));
@@ -294,11 +295,11 @@ int ent_mlp_create(ent_ctx* ctx, int n_in, int hidden, int n_hidden_layers, uint
 
 int ent_mlp_destroy(ent_mlp* M) {
   if (!M) return ENT_OK;
-  hipSetDevice(M->device);
-  hipDeviceSynchronize();
+  (void)hipSetDevice(M->device);
+  (void)hipDeviceSynchronize();
   for (void* p : {(void*)M->p, (void*)M->g, (void*)M->m, (void*)M->v, (void*)M->p16, (void*)M->act,
                   (void*)M->dact, (void*)M->xb, (void*)M->tb, (void*)M->loss, (void*)M->idx})
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   delete M;
   return ENT_OK;
 }
