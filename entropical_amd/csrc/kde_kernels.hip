@@ -132,13 +132,8 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // touching every active job.
 constexpr uint32_t kXcds = 8;
 __device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t G) {
-#ifdef ENT_NO_XCD_ORDER
-  (void)G;
-  return b;
-#else
   const uint32_t q = G / kXcds, r = G % kXcds, x = b % kXcds;
   return x * q + (x < r ? x : r) + b / kXcds;
-#endif
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -1326,14 +1321,12 @@ __device__ __forceinline__ void run_tile(const DJob& J, const DTile& T, const fl
                                          double* __restrict__ dens, WaveLds& L, uint64_t& pairs,
                                          uint64_t& lane_slots) {
   const uint32_t nq = T.p1 - T.p0;
-#ifndef ENT_STATIC_Q
   if constexpr (QMAX >= 2 && !RAW) {  // plain-path tiles are full except the last
     if (nq <= (uint32_t)(kWave * (QMAX - 1))) {
       run_tile<D, QMAX - 1, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, pairs, lane_slots);
       return;
     }
   }
-#endif
   const uint32_t total = process_tile<D, QMAX, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L);
   pairs += (uint64_t)nq * total;
   lane_slots += (uint64_t)(kWave * QMAX) * ((total + kWave - 1) / kWave * kWave);
