@@ -91,7 +91,8 @@ int Engine::autotune(const float* X, const std::vector<JobSpec>& jobs, double* d
   // m0 * group_h its extent along dim 0, mr the row height), then the four
   // fastest re-timed on the whole batch: differences of a few percent are
   // below the subsample's run-to-run noise.
-  const std::vector<int> qs = {1, 2, 4}, m0s = {4, 8, 16}, mrs = {1, 2}, ghs = {2, 4, 8};
+  // (group_h = 64 spans the whole range of standardised data: no split)
+  const std::vector<int> qs = {1, 2, 4}, m0s = {8, 16, 32}, mrs = {1, 2}, ghs = {4, 16, 64};
   std::vector<std::pair<double, Tuning>> scored;
   for (int q : qs)
     for (int m0 : m0s)

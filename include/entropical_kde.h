@@ -226,7 +226,7 @@ int ent_hestimate_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_
 
 /* Tile-shape autotuner (no reference counterpart; the reference fixes its
  * tile at compile time, densities_cuda.cu:14).  Times every setting of the
- * grid q in {1,2,4}, m0 in {4,8,16}, mr in {1,2}, group_h in {2,4,8} on a
+ * grid q in {1,2,4}, m0 in {8,16,32}, mr in {1,2}, group_h in {4,16,64} on a
  * subsample of the job set (the four fastest re-timed on the whole set),
  * keeps the fastest on the context (and returns it in tuning4 when non-NULL),
  * and caches it per workload signature (file named by ENT_TUNE_CACHE, if set).
