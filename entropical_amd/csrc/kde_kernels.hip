@@ -850,6 +850,11 @@ __global__ void k_tile_pack(const DJob* __restrict__ jobs, DTile* __restrict__ t
       const DTile first = tiles[a];
       const DJob& J = jobs[first.job];
       if (J.d < 2 || (J.flags & kJobBrute)) return false;
+      // a sub-tile (one row) needs up to (2 rows per h + 1)^(d-1) candidate
+      // rows, one per lane of its group of 64 / cnt lanes
+      uint32_t rows = 1;
+      for (uint32_t e = 1; e < J.d; ++e) rows *= 2u * (uint32_t)rintf(J.h[e] / J.bs[e]) + 1u;
+      if (rows > kWave / cnt) return false;
       for (uint32_t i = a; i < a + cnt; ++i) {
         const DTile ti = tiles[i];
         if (ti.job != first.job || ti.p1 - ti.p0 > lim) return false;
