@@ -113,6 +113,10 @@ int Engine::densities(const float* X, const std::vector<JobSpec>& jobs, double* 
       set_error("job with zero points");
       return kErrArg;
     }
+    if (j.n >= (1u << 30)) {  // the density kernel addresses a job's columns with 32-bit byte offsets
+      set_error("job exceeds 2^30 points");
+      return kErrUnsupported;
+    }
     for (uint32_t e = 0; e < j.d; ++e)
       if (!(j.h[e] > 0.0f) || !std::isfinite(j.h[e])) {
         set_error("bandwidths must be finite and > 0");

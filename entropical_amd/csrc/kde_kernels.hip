@@ -14,6 +14,7 @@
 #define ENT_DENSITY_MIN_WAVES 1
 #endif
 
+
 namespace ent {
 namespace {
 
@@ -1130,22 +1131,35 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
   // starting at or before the previous chunk, and stops at the first range
   // starting past this chunk: a chunk visits the ranges that start inside it.
   uint32_t kc = 0;
+  // range kc's offset and the next range's start, kept between chunks: a
+  // chunk that ends before the next range starts needs no scan
+  uint32_t kc_off = __builtin_amdgcn_readlane(my_lo, 0) - __builtin_amdgcn_readlane(my_ps, 0);
+  uint32_t next_ps = n_ranges > 1 ? __builtin_amdgcn_readlane(my_ps, 1) : 0xFFFFFFFFu;
   auto load_cand = [&](uint32_t cb, float* x) {
     const uint32_t f = cb + (uint32_t)l;
     const uint32_t cend = cb + kWave;
     const bool cend_wraps = cend < cb;
     uint32_t off = 0;  // sorted position - flattened index, mod 2^32
-    for (uint32_t k = kc; k < n_ranges; ++k) {
-      const uint32_t ps_k = __builtin_amdgcn_readlane(my_ps, k);
-      if (!cend_wraps && ps_k >= cend) break;
-      const uint32_t lo_k = __builtin_amdgcn_readlane(my_lo, k);
-      if (ps_k <= cb) kc = k;
-      if (f >= ps_k) off = lo_k - ps_k;
+    if (!cend_wraps && next_ps >= cend) {
+      off = kc_off;
+    } else {
+      for (uint32_t k = kc; k < n_ranges; ++k) {
+        const uint32_t ps_k = __builtin_amdgcn_readlane(my_ps, k);
+        if (!cend_wraps && ps_k >= cend) break;
+        const uint32_t lo_k = __builtin_amdgcn_readlane(my_lo, k);
+        if (ps_k <= cb) kc = k;
+        if (f >= ps_k) off = lo_k - ps_k;
+      }
+      kc_off = __builtin_amdgcn_readlane(my_lo, kc) - __builtin_amdgcn_readlane(my_ps, kc);
+      next_ps = kc + 1 < n_ranges ? __builtin_amdgcn_readlane(my_ps, kc + 1) : 0xFFFFFFFFu;
     }
     if (f < total) {
       const uint32_t p = f + off;
+      // 32-bit byte offsets from the dims' uniform base addresses (a job
+      // holds fewer than 2^30 points, so a coordinate column is below 4 GiB)
 #pragma unroll
-      for (int e = 0; e < D; ++e) x[e] = src[e][p];
+      for (int e = 0; e < D; ++e)
+        x[e] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(src[e]) + (uint32_t)(p * 4u));
     } else {
 #pragma unroll
       for (int e = 0; e < D; ++e) x[e] = kSentinel;
