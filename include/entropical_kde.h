@@ -28,7 +28,9 @@ extern "C" {
 #define ENT_ERR_HIP 3         /* HIP runtime error (replaces check_error, tools.cuh:33-42) */
 #define ENT_ERR_OOM 4
 #define ENT_ERR_NONFINITE 5   /* NaN/Inf coordinate */
-#define ENT_ERR_UNSUPPORTED 6 /* configuration outside the engine's limits */
+#define ENT_ERR_UNSUPPORTED 6 /* configuration outside the engine's limits: a job of 2^30
+                                 points or more, a batch of 2^32 points or more, more than
+                                 2^24 boxes along one dim */
 
 #define ENT_METHOD_CELL 0     /* cell-list candidates (default) */
 #define ENT_METHOD_BRUTE 1    /* all pairs, as the reference's CUDA backend does */
