@@ -170,8 +170,11 @@ void build_te(size_t N, size_t C, const float* h, const uint32_t* taus, size_t n
       for (uint32_t j = i + 1; j < C; ++j) {
         off_itau[t][i * C + j] = alloc(N - tau);
         jobs.push_back(make_job(N, {i, i, j}, {h[i], h[i], h[j]}, {0, tau, 0}, off_itau[t][i * C + j]));
+        // p(x_t, y_t, y_t+tau) with its dims listed as (y_t, y_t+tau, x_t): the
+        // same density, and its dims 0 and 1 then match those of the other
+        // jobs of series j and lag tau, so they share one two-dim sort
         off_jtau[t][i * C + j] = alloc(N - tau);
-        jobs.push_back(make_job(N, {i, j, j}, {h[i], h[j], h[j]}, {0, 0, tau}, off_jtau[t][i * C + j]));
+        jobs.push_back(make_job(N, {j, j, i}, {h[j], h[j], h[i]}, {0, tau, 0}, off_jtau[t][i * C + j]));
       }
   }
   for (uint32_t i = 0; i < C; ++i)

@@ -297,32 +297,45 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   // Two-stage sort: jobs with the same dim-0 window (offset, length) and
   // dim-0 bandwidth have the same dim-0 boxes; one stable sort of that window
   // by dim-0 box (a base) serves all of them, and each job then needs only
-  // the passes over its remaining dims' key (k_major_keys).  Chosen when the
-  // bases hold at most half the batch's points; ENT_SORT=full|shared forces
-  // a mode (both give the same permutation).
+  // the passes over its remaining dims' key (k_major_keys).  3-D jobs that
+  // share their dims 0 and 1 with another job (the TE set: one pair of lagged
+  // windows per series and lag) get a base sorted by those two dims and one
+  // pass over dim 2.  Chosen when the bases hold at most half the batch's
+  // points; ENT_SORT=full|shared forces a mode (all give the same permutation).
   h_bjobs.clear();
   h_bchunks.clear();
   h_bchunk_first.clear();
   h_bbase.assign(J, 0);
   uint64_t PB = 0, max_b0 = 0, max_major = 0;
   {
-    std::map<std::tuple<uint64_t, uint32_t, uint32_t>, uint32_t> seen;
+    // (off0, off1, n, h0 bits, h1 bits); off1 = ~0 for a dim-0 base
+    using BaseKey = std::tuple<uint64_t, uint64_t, uint32_t, uint32_t, uint32_t>;
+    auto key_of = [&](const DJob& D, bool two) {
+      uint32_t h0, h1 = 0;
+      std::memcpy(&h0, &D.h[0], 4);
+      if (two) std::memcpy(&h1, &D.h[1], 4);
+      return BaseKey(D.off[0], two ? D.off[1] : ~0ull, D.n, h0, h1);
+    };
+    std::map<BaseKey, uint32_t> count2;
+    for (uint32_t j = 0; j < J; ++j)
+      if (h_jobs[j].d == 3) ++count2[key_of(h_jobs[j], true)];
+    std::map<BaseKey, uint32_t> seen;
     for (uint32_t j = 0; j < J; ++j) {
-      const DJob& D = h_jobs[j];
-      uint32_t hbits;
-      std::memcpy(&hbits, &D.h[0], 4);
-      auto ins = seen.emplace(std::make_tuple(D.off[0], D.n, hbits), (uint32_t)h_bjobs.size());
+      DJob& D = h_jobs[j];
+      const bool two = D.d == 3 && count2[key_of(D, true)] >= 2;
+      if (two) D.flags |= kJobBase2;
+      auto ins = seen.emplace(key_of(D, two), (uint32_t)h_bjobs.size());
       if (ins.second) {
         DJob B = D;
-        B.d = 1;
-        for (int e = 1; e < 3; ++e) {
+        B.d = two ? 2 : 1;
+        for (int e = (int)B.d; e < 3; ++e) {
           B.off[e] = 0;
           B.h[e] = B.hinv[e] = B.bs[e] = 1.0f;
           B.mn[e] = 0.0f;
           B.nb[e] = 1;
         }
         B.flags = 0;
-        B.cells = D.nb[0];
+        B.cells = two ? (uint64_t)D.nb[0] * D.nb[1] : D.nb[0];
         B.cbase = kNoTable;
         B.pbase = PB;
         const uint32_t bi = (uint32_t)h_bjobs.size();
@@ -332,8 +345,8 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
         h_bjobs.push_back(B);
       }
       h_bbase[j] = h_bjobs[ins.first->second].pbase;
-      max_b0 = std::max<uint64_t>(max_b0, D.nb[0] - 1);
-      max_major = std::max<uint64_t>(max_major, D.cells / D.nb[0] - 1);
+      max_b0 = std::max<uint64_t>(max_b0, h_bjobs[ins.first->second].cells - 1);
+      max_major = std::max<uint64_t>(max_major, two ? D.nb[2] - 1 : D.cells / D.nb[0] - 1);
     }
     h_bchunk_first.push_back((uint32_t)h_bchunks.size());
   }

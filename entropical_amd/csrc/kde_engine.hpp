@@ -29,6 +29,8 @@ constexpr int kMaxRanges = 64;     // candidate row ranges per tile (one per lan
 constexpr uint32_t kJobBrute = 1u;  // all-pairs over the job's points (plain path)
 constexpr uint32_t kJobTri = 2u;    // 1-D triangular kernel max(0, 1-|u|): the
                                     // Epanechnikov self-convolution of epa_convolution
+constexpr uint32_t kJobBase2 = 8u;  // two-stage sort of a 3-D job: its base is sorted by
+                                    // dims 0 and 1, its own passes are over dim 2
 
 // Device job descriptor.  Points of job j, dim e, index k in [0, n):
 //   X[off[e] + k].

@@ -527,11 +527,14 @@ __device__ __forceinline__ void major_keys_chunk(const DJob& J, uint32_t y, cons
       id[i] = bi[k < last ? k : last];
     }
     float v[kB][3];
+    // over a base sorted by dims 0 and 1, only dim 2 is needed
+    const int e0 = (!FINAL && D == 3 && (J.flags & kJobBase2)) ? 2 : E0;
 #pragma unroll
     for (int i = 0; i < kB; ++i) {
       v[i][0] = v[i][1] = v[i][2] = 0.0f;
 #pragma unroll
-      for (int e = E0; e < D; ++e) v[i][e] = X[J.off[e] + id[i]];
+      for (int e = E0; e < D; ++e)
+        if (e >= e0) v[i][e] = X[J.off[e] + id[i]];
     }
 #pragma unroll
     for (int i = 0; i < kB; ++i) {
@@ -543,10 +546,14 @@ __device__ __forceinline__ void major_keys_chunk(const DJob& J, uint32_t y, cons
         for (int e = 0; e < D; ++e) scoord[3 * J.pbase + (uint64_t)e * J.n + k] = v[i][e];
         keys[J.pbase + k] = (KeyT)cell_key(J, v[i]);
       } else {
+        // key of the dims the base does not order: 1.. (or 2 over a base
+        // sorted by dims 0 and 1)
         uint64_t r = 0;
-        if constexpr (D >= 2) {
+        if constexpr (D == 3) {
+          r = box_of(v[i][2], J.mn[2], J.bs[2], J.nb[2]);
+          if (!(J.flags & kJobBase2)) r = r * J.nb[1] + box_of(v[i][1], J.mn[1], J.bs[1], J.nb[1]);
+        } else if constexpr (D == 2) {
           r = box_of(v[i][1], J.mn[1], J.bs[1], J.nb[1]);
-          if constexpr (D == 3) r += (uint64_t)box_of(v[i][2], J.mn[2], J.bs[2], J.nb[2]) * J.nb[1];
         }
         keys[J.pbase + k] = (KeyT)r;
       }
