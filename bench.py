@@ -13,12 +13,16 @@ Other workloads (--workload): te (32 coupled AR(1) series x 100,000 frames,
 h=0.2, lags 1,2,5,10), uneven (500 trajectories of 2,000-500,000 frames,
 d 1-3), entropy1d (N=100,000 three-component mixture, h=0.1).
 
-Printed JSON line: value = executed pair evaluations per second
-(Gpair-evals/s) over the whole step; the dense-equivalent rate, the plain
+Printed JSON line: value = candidate pair evaluations per second
+(Gpair-evals/s) over the whole step: the (query, candidate) pairs of every
+tile's cell-list window, evaluated one by one or, for the boxes of a 1-D job
+that lie inside the support of every query of a tile, summed exactly from
+the box's moments (both counts are reported); the dense-equivalent rate, the plain
 per-pair GPU path (the reference's per-call driver order, extrapolated from
 timed calls) and the host OpenMP baseline (oracle restatement of
 densities_omp.cpp on timed calls, extrapolated) are reported beside it.
-Roofline: FP32 VALU; achieved = algorithmic FLOPs (4d per executed pair) /
+Roofline: FP32 VALU; achieved = algorithmic FLOPs (4d per pair evaluated one
+by one, 6 per (query, box) moment term) /
 duration of the persistent density kernel, measured with HIP events on the
 engine's stream.  The tile shape is chosen by the engine's autotuner during
 warmup (untimed).
@@ -35,6 +39,9 @@ import numpy as np
 
 PEAK_FP32_TFLOPS = 157.3
 FLOPS_PER_PAIR = (4, 8, 12)  # d = 1, 2, 3
+# (query, box) term of a 1-D box inside every query's support, which stands
+# for all of the box's pairs: d = x - r, t = B - A d, c + t d, accumulate
+MOMENT_TERM_FLOPS = 6
 
 
 def log(msg: str) -> None:
@@ -81,8 +88,18 @@ def measured_traffic(workload: str):
         return None
 
 
-def roofline(pairs_by_d, kern_ms: float, workload: str) -> dict:
-    flops = sum(p * f for p, f in zip(pairs_by_d, FLOPS_PER_PAIR))
+def evaluated_pairs(st) -> list:
+    """Pairs evaluated one by one per job dimension: the candidate pairs less
+    the 1-D pairs summed from box moments."""
+    by_d = list(st["pairs_by_d"])
+    by_d[0] -= st["moment_pairs"]
+    return by_d
+
+
+def roofline(st, kern_ms: float, workload: str) -> dict:
+    """Algorithmic FLOP rate of k_density: 4d FLOPs per pair evaluated one by
+    one plus MOMENT_TERM_FLOPS per (query, box) moment term."""
+    flops = sum(p * f for p, f in zip(evaluated_pairs(st), FLOPS_PER_PAIR)) + st["moment_terms"] * MOMENT_TERM_FLOPS
     achieved = flops / (kern_ms * 1e-3) / 1e12
     return {
         "bound": "valu",
@@ -93,7 +110,8 @@ def roofline(pairs_by_d, kern_ms: float, workload: str) -> dict:
         "frac": achieved / PEAK_FP32_TFLOPS,
         "traffic": measured_traffic(workload),
         "kernel_ms": kern_ms,
-        "flops_per_pair": "4d (d=1: 4, d=2: 8, d=3: 12)",
+        "flops_per_pair": "4d per pair evaluated one by one (d=1: 4, d=2: 8, d=3: 12); "
+                          "6 per (query, box) term of 1-D box moments",
     }
 
 
@@ -127,7 +145,7 @@ def timed_steps(ctx, step, steps: int, warmup: int):
         step()
         st = ctx.last_stats()
         dens_ms.append(st["density_ms"])
-        util.append(sum(st["pairs_by_d"]) / max(1, st["lane_pairs"]))
+        util.append(sum(evaluated_pairs(st)) / max(1, st["lane_pairs"]))
     ctx.sync()
     elapsed = time.perf_counter() - t0
     return elapsed / steps, float(np.mean(dens_ms)), st, float(np.mean(util))
@@ -168,11 +186,11 @@ def run_mi(args) -> dict:
             "n_cols": n_cols, "n_rows": n_rows, "bandwidth": h, "method": "cell list",
             "tuning_q_m0_mr_group": list(tun),
         },
-        "pairs_executed": pairs,
+        "pairs_candidate": pairs, "pairs_box_moments": st["moment_pairs"], "moment_terms": st["moment_terms"],
         "pairs_by_dim": st["pairs_by_d"],
         "lane_utilisation": util,
         "dense_equiv_gpair_per_s": 2080 * n_rows * n_rows / sec / 1e9,
-        "roofline": roofline(st["pairs_by_d"], kern_ms, "mi"),
+        "roofline": roofline(st, kern_ms, "mi"),
     }
     if not args.no_plain:
         log("timing plain per-pair path (one call each, extrapolated to mi.cpp's 4096 calls)")
@@ -227,9 +245,9 @@ def run_te(args) -> dict:
         "config": {"workload": "te_32series_100000frames_h0.2_tau1-2-5-10",
                    "jobs": f"{jobs_3d} 3-D + {jobs_2d} 2-D + {jobs_1d} 1-D densities, one persistent launch",
                    "tuning_q_m0_mr_group": list(tun)},
-        "pairs_executed": pairs, "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
+        "pairs_candidate": pairs, "pairs_box_moments": st["moment_pairs"], "moment_terms": st["moment_terms"], "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
         "dense_equiv_gpair_per_s": dense / sec / 1e9,
-        "roofline": roofline(st["pairs_by_d"], kern_ms, "te"),
+        "roofline": roofline(st, kern_ms, "te"),
     }
     # the reference runs one lag per invocation: per lag 32 1-D + 32 lagged
     # 2-D + 496 2-D + 992 3-D combined_densities calls (transs.cpp:42-74)
@@ -283,9 +301,9 @@ def run_uneven(args) -> dict:
         "value": pairs / sec / 1e9, "unit": "Gpair-evals/s", "ms_per_step": 1e3 * sec,
         "config": {"workload": "uneven_500traj_2k-500k_d1-3", "points": int(sum(j["n"] for j in jobs)),
                    "tuning_q_m0_mr_group": list(tun)},
-        "pairs_executed": pairs, "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
+        "pairs_candidate": pairs, "pairs_box_moments": st["moment_pairs"], "moment_terms": st["moment_terms"], "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
         "dense_equiv_gpair_per_s": dense / sec / 1e9,
-        "roofline": roofline(st["pairs_by_d"], kern_ms, "uneven"),
+        "roofline": roofline(st, kern_ms, "uneven"),
     }
 
 
@@ -307,8 +325,8 @@ def run_entropy1d(args) -> dict:
         "metric": "Gpair-evals/s (1-D KDE entropy, N=100k)", "value": pairs / sec / 1e9, "unit": "Gpair-evals/s",
         "ms_per_step": 1e3 * sec,
         "config": {"workload": "entropy1d_gmm3_100000_h0.1", "tuning_q_m0_mr_group": list(tun)},
-        "entropy": H, "pairs_executed": pairs, "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
-        "dense_equiv_gpair_per_s": float(n) ** 2 / sec / 1e9, "roofline": roofline(st["pairs_by_d"], kern_ms, "entropy1d"),
+        "entropy": H, "pairs_candidate": pairs, "pairs_box_moments": st["moment_pairs"], "moment_terms": st["moment_terms"], "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
+        "dense_equiv_gpair_per_s": float(n) ** 2 / sec / 1e9, "roofline": roofline(st, kern_ms, "entropy1d"),
     }
     if not args.no_plain:
         combined_densities(x, n, [0], [h], method="brute", ctx=ctx)

@@ -59,7 +59,7 @@ Engine::~Engine() {
                     &d_keys[1], &d_idx[0], &d_idx[1],     &d_hist,   &d_scan_scratch, &d_scoord,
                     &d_tile_cnt, &d_tiles, &d_counter, &d_cellstart, &d_bjobs, &d_bchunks,
                     &d_bchunk_first, &d_bbase, &d_bkeys[0], &d_bkeys[1], &d_bidx[0], &d_bidx[1],
-                    &d_wjobs, &d_wchunks, &d_tchunks};
+                    &d_wjobs, &d_wchunks, &d_tchunks, &d_moments, &d_mlist};
   for (GraphCache* c : {&pre_graphs_, &dens_graphs_})
     for (auto& e : c->entries) (void)hipGraphExecDestroy(e.second);
   for (DevBuf* b : bufs)
@@ -283,6 +283,38 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     }
     max_key = std::max<uint64_t>(max_key, cells - 1);
   }
+  // Box moments for 1-D Epanechnikov jobs with a cell-start table and no more
+  // boxes than points (on unless ENT_BOX_MOMENTS=0): each job's boxes get a
+  // moment table, listed for k_box_moments in waves of kMomentBoxesPerWave
+  // boxes.
+  bool moments = true;
+  if (const char* m = std::getenv("ENT_BOX_MOMENTS"))
+    if (!std::strcmp(m, "0")) moments = false;
+  uint64_t mom_total = 0;
+  h_mlist.assign(1, 0u);
+  {
+    std::vector<uint32_t> mj;
+    uint32_t waves = 0;
+    for (uint32_t j = 0; j < J; ++j) {
+      DJob& D = h_jobs[j];
+      D.flags &= ~kJobMoments;
+      D.mbase = 0;
+      if (!moments || D.d != 1 || (D.flags & kJobTri) || D.cbase == kNoTable || D.nb[0] > D.n) continue;
+      D.flags |= kJobMoments;
+      D.mbase = mom_total;
+      mom_total += D.nb[0];
+      mj.push_back(j);
+      h_mlist.push_back(waves);
+      waves += (D.nb[0] + kMomentBoxesPerWave - 1) / kMomentBoxesPerWave;
+    }
+    h_mlist[0] = (uint32_t)mj.size();
+    h_mlist.push_back(waves);
+    h_mlist.insert(h_mlist.end(), mj.begin(), mj.end());
+  }
+  const bool has_mom = mom_total != 0;
+  // grid bound of k_box_moments from the plan's shape: the listed boxes are at
+  // most the points
+  const uint32_t mom_waves = (uint32_t)std::min<uint64_t>(P / kMomentBoxesPerWave + J, 1u << 30);
   auto passes_for = [](uint64_t max_value) {
     int bits = 0;
     while (bits < 64 && (max_value >> bits) != 0) ++bits;
@@ -409,12 +441,20 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   const size_t base_bytes = shared ? sizeof(DJob) * NB + sizeof(uint2) * NCB + sizeof(uint32_t) * (NB + 1) +
                                          sizeof(uint64_t) * J
                                    : 0;
-  ENT_TRY(ensure_host(sizeof(DJob) * J + base_bytes + (upload_tchunks ? sizeof(uint2) * NC : 0) + 64));
+  const size_t mlist_bytes = sizeof(uint32_t) * h_mlist.size();
+  ENT_TRY(ensure_host(sizeof(DJob) * J + base_bytes + (upload_tchunks ? sizeof(uint2) * NC : 0) + mlist_bytes +
+                      64));
   {
     char* p = (char*)host_stage_;
     std::memcpy(p, h_jobs.data(), sizeof(DJob) * J);
     ENT_HIP(hipMemcpyAsync(d_jobs.p, p, sizeof(DJob) * J, hipMemcpyHostToDevice, stream_));
     p += sizeof(DJob) * J;
+    if (has_mom) {
+      ENT_TRY(ensure(d_mlist, mlist_bytes));
+      std::memcpy(p, h_mlist.data(), mlist_bytes);
+      ENT_HIP(hipMemcpyAsync(d_mlist.p, p, mlist_bytes, hipMemcpyHostToDevice, stream_));
+      p += (mlist_bytes + 7) & ~(size_t)7;
+    }
     if (upload_tchunks) {
       std::memcpy(p, h_tchunks.data(), sizeof(uint2) * NC);
       ENT_HIP(hipMemcpyAsync(d_tchunks.p, p, sizeof(uint2) * NC, hipMemcpyHostToDevice, stream_));
@@ -455,6 +495,7 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   ENT_TRY(ensure(d_tiles, sizeof(DTile) * P));
   ENT_TRY(ensure(d_counter, sizeof(uint32_t) * kCounterWords));
   ENT_TRY(ensure(d_cellstart, sizeof(uint32_t) * std::max<uint64_t>(table_total, 1)));
+  if (has_mom) ENT_TRY(ensure(d_moments, sizeof(float4) * mom_total));
   if (shared) {
     for (int b = 0; b < 2; ++b) {
       ENT_TRY(ensure(d_bkeys[b], key_bytes * PB));
@@ -509,6 +550,9 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     }
     if (!with_tiles) return;
     if (table_total) launch_cell_starts(stream_, wide, dj, dch, NC, keys[cur], (uint32_t*)d_cellstart.p);
+    if (has_mom)
+      launch_box_moments(stream_, dj, (const uint32_t*)d_mlist.p, mom_waves, scoord, (const uint32_t*)d_cellstart.p,
+                         (float4*)d_moments.p);
     const uint2* tch = (const uint2*)d_tchunks.p;
     launch_tile_count(stream_, wide, dj, tch, NC, keys[cur], tq, tcnt);
     launch_exclusive_scan(stream_, tcnt, tcnt, NC, scratch);
@@ -524,7 +568,7 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   PlanKey key = plan_key(X, dens, J, NC, P, npass, false);
   key.NCB = shared ? NCB : 0;
   key.PB = shared ? PB : 0;
-  key.variant = (shared ? 1 : 0) | (staged ? 2 : 0) | (pack ? 4 : 0) | (npass0 << 3);
+  key.variant = (shared ? 1 : 0) | (staged ? 2 : 0) | (pack ? 4 : 0) | (npass0 << 3) | (has_mom ? 256 : 0);
   ENT_TRY(run_captured(pre_graphs_, key, [&] { issue_pre(true); }));
   return launch_density_common(false, X, dens, P);
 }
@@ -645,8 +689,8 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
     rc = launch_density(stream_, tuning.q, raw, wide, grid, (const DJob*)d_jobs.p, (const DTile*)d_tiles.p,
                         n_tiles, counter, X, raw ? nullptr : (const float*)d_scoord.p,
                         raw ? nullptr : d_keys[sorted_buf].p, raw ? nullptr : (const uint32_t*)d_cellstart.p,
-                        raw ? nullptr : (const uint32_t*)d_idx[sorted_buf].p, dens,
-                        (unsigned long long*)(counter + 2));
+                        raw ? nullptr : (const uint32_t*)d_idx[sorted_buf].p,
+                        raw ? nullptr : (const float4*)d_moments.p, dens, (unsigned long long*)(counter + 2));
   };
   ENT_HIP(hipEventRecord(ev_[1], stream_));
   if (raw) {
@@ -666,15 +710,17 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
   // stats readback
   ENT_TRY(ensure_host(64));
   uint32_t* hc = (uint32_t*)host_stage_;
-  ENT_HIP(hipMemcpyAsync(hc, counter, 40, hipMemcpyDeviceToHost, stream_));
-  ENT_HIP(hipMemcpyAsync(hc + 10, n_tiles, 4, hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipMemcpyAsync(hc, counter, 56, hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipMemcpyAsync(hc + 14, n_tiles, 4, hipMemcpyDeviceToHost, stream_));
   ENT_HIP(hipStreamSynchronize(stream_));
-  uint64_t pd[4];
-  std::memcpy(pd, hc + 2, 32);
+  uint64_t pd[6];
+  std::memcpy(pd, hc + 2, 48);
   last.lane_pairs = pd[3];
   last.pairs = pd[0] + pd[1] + pd[2];
   for (int d = 0; d < 3; ++d) last.pairs_by_d[d] = pd[d];
-  last.tiles = hc[10];
+  last.moment_pairs = pd[4];
+  last.moment_terms = pd[5];
+  last.tiles = hc[14];
   float ms = 0.f;
   ENT_HIP(hipEventElapsedTime(&ms, ev_[1], ev_[2]));
   last.density_ms = ms;

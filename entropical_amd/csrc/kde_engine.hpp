@@ -31,6 +31,9 @@ constexpr uint32_t kJobTri = 2u;    // 1-D triangular kernel max(0, 1-|u|): the
                                     // Epanechnikov self-convolution of epa_convolution
 constexpr uint32_t kJobBase2 = 8u;  // two-stage sort of a 3-D job: its base is sorted by
                                     // dims 0 and 1, its own passes are over dim 2
+constexpr uint32_t kJobMoments = 16u;  // 1-D job with a box-moment table: the boxes that lie
+                                       // inside every query's support are summed from
+                                       // their moments instead of pair by pair
 
 // Device job descriptor.  Points of job j, dim e, index k in [0, n):
 //   X[off[e] + k].
@@ -51,6 +54,7 @@ struct DJob {
   double norm;       // prod_e(0.75/h_e) / n_rows
   uint64_t cells;    // prod_e nb[e]
   uint64_t cbase;    // first entry of this job's cell-start table, or kNoTable
+  uint64_t mbase;    // first entry of this job's box-moment table (kJobMoments)
 };
 
 // Jobs whose cell count exceeds this multiple of their points (sparse grids,

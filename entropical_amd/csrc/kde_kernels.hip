@@ -736,6 +736,68 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
   }
 }
 
+// ------------------------------------------------------------ box moments
+// Moment table of the 1-D jobs flagged kJobMoments.  Box b of job J, with
+// origin r_b = fma(b, bs, mn) and the offsets e_j = x_j - r_b of its s0
+// points (summed in FP64), stores
+//   A = s0 / h^2,  B = 2 S1 / h^2,  c = s0 - S2 / h^2   (S1 = sum e, S2 = sum e^2)
+// so that for a query x with d = x - r_b whose support holds the whole box,
+//   sum_j (1 - ((x - x_j) / h)^2) = c + d (B - A d).
+// mlist: [0] = M jobs, [1 .. M+1] = first wave of each job (prefix), then
+// the M job indices; each wave covers kMomentBoxes consecutive boxes of one
+// job.  The grid strides over the waves.
+constexpr uint32_t kMomentBoxes = kMomentBoxesPerWave;
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_box_moments(const DJob* __restrict__ jobs,
+                                                        const uint32_t* __restrict__ mlist,
+                                                        const float* __restrict__ scoord,
+                                                        const uint32_t* __restrict__ start,
+                                                        float4* __restrict__ mom) {
+  const uint32_t M = mlist[0];
+  const uint32_t* wfirst = mlist + 1;
+  const uint32_t* mjob = mlist + 2 + M;
+  const uint32_t n_waves = wfirst[M];
+  const int l = lane_id();
+  const uint32_t stride = gridDim.x * (kBlock / kWave);
+  for (uint32_t w = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; w < n_waves; w += stride) {
+    // the job holding wave w: last k with wfirst[k] <= w
+    uint32_t lo = 0, hi = M;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (wfirst[mid] <= w) lo = mid;
+      else hi = mid;
+    }
+    const DJob& J = jobs[mjob[lo]];
+    const float* x = scoord + 3 * J.pbase;
+    const uint32_t* st = start + J.cbase;
+    const uint32_t b0 = (w - wfirst[lo]) * kMomentBoxes;
+    const uint32_t b1 = min(b0 + kMomentBoxes, J.nb[0]);
+    const double hi2 = (double)J.hinv[0] * (double)J.hinv[0];
+    for (uint32_t b = b0; b < b1; ++b) {
+      const uint32_t a = st[b], e = st[b + 1];
+      const float r = fmaf((float)b, J.bs[0], J.mn[0]);
+      double s1 = 0.0, s2 = 0.0;
+      for (uint32_t p = a + (uint32_t)l; p < e; p += kWave) {
+        const double ej = (double)x[p] - (double)r;
+        s1 += ej;
+        s2 = fma(ej, ej, s2);
+      }
+      s1 = wave_sum_f64(s1);
+      s2 = wave_sum_f64(s2);
+      if (l == 0) {
+        const double s0 = (double)(e - a);
+        mom[J.mbase + b] = make_float4((float)(s0 * hi2), (float)(2.0 * s1 * hi2), (float)(s0 - s2 * hi2), 0.0f);
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ tiles
 // A tile starts at sorted position p when p == 0, p % tq == 0, or the
 // (row, dim-0 group) of p differs from that of p-1.  Tiles never cross rows,
@@ -1018,13 +1080,57 @@ __device__ __forceinline__ void accumulate_chunk(const float4* c4, const float (
   for (int g = 0; g < kReads; ++g) consume(c4[g], g);
 }
 
-// Densities of one tile's queries; returns the tile's candidate count
-// (wave-uniform).
+// Moment boxes [b0, b0 + nbx) of a 1-D tile added into the lanes' queries:
+// 64 boxes at a time, lane k stages box k's origin and moments in LDS, then
+// every lane adds c + d (B - A d), d = xq - r, per box (padding boxes are
+// zero), with a compensated merge every 8 boxes.
+template <int Q>
+__device__ __forceinline__ void moment_boxes(const DJob& J, const float4* __restrict__ mom, uint32_t b0,
+                                             uint32_t nbx, const float (&xq)[3][Q], float (&S)[Q],
+                                             float (&C)[Q], WaveLds& L) {
+  const int l = lane_id();
+  for (uint32_t base = 0; base < nbx; base += kWave) {
+    const uint32_t cnt = min(nbx - base, (uint32_t)kWave);
+    float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
+    float r = 0.f;
+    if ((uint32_t)l < cnt) {
+      const uint32_t b = b0 + base + (uint32_t)l;
+      m = mom[J.mbase + b];
+      r = fmaf((float)b, J.bs[0], J.mn[0]);
+    }
+    wave_sync();
+    L.cand[l] = make_float4(r, m.x, m.y, m.z);
+    wave_sync();
+    const uint32_t cnt8 = (cnt + 7u) & ~7u;
+    for (uint32_t k = 0; k < cnt8; k += 8) {
+      float part[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) part[q] = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const float4 v = L.cand[k + i];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const float d = xq[0][q] - v.x;
+          const float t = __builtin_fmaf(-v.y, d, v.z);
+          part[q] += __builtin_fmaf(t, d, v.w);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < Q; ++q) kahan_add(S[q], C[q], part[q]);
+    }
+  }
+}
+
+// Densities of one tile's queries; returns the tile's pair-by-pair candidate
+// count (wave-uniform).  mstat (1-D jobs with a moment table): candidates
+// summed from box moments and the number of those boxes.
 template <int D, int Q, bool RAW, typename KeyT>
 __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __restrict__ X,
                                  const float* __restrict__ scoord, const KeyT* __restrict__ keys,
                                  const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ idx,
-                                 double* __restrict__ dens, WaveLds& L) {
+                                 const float4* __restrict__ mom, double* __restrict__ dens, WaveLds& L,
+                                 uint32_t (&mstat)[2]) {
   const int l = lane_id();
   const uint32_t n = J.n;
   const float* src[3];
@@ -1052,6 +1158,7 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
   float o[3] = {0.f, 0.f, 0.f};
   uint32_t n_ranges = 1;
   uint32_t my_lo = 0, my_len = 0;
+  uint32_t m_lo = 0, m_n = 0;  // boxes summed from the moment table
   if constexpr (RAW) {
     if (l == 0) { my_lo = 0; my_len = n; }
   } else {
@@ -1072,11 +1179,36 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
     uint32_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 #pragma unroll
     for (int e = 0; e < D; ++e) box_range(bmin[e], bmax[e], J.h[e], J.mn[e], J.bs[e], J.nb[e], lo[e], hi[e]);
+    if constexpr (D == 1) {
+      if (J.flags & kJobMoments) {
+        // Boxes [ilo, ihi] hold only points x with bmax - h < x < bmin + h
+        // (box_of is monotone and the margin covers the rounding of the box
+        // arithmetic), i.e. inside the support of every query of the tile;
+        // they are summed from their moments, the boxes at either end of the
+        // candidate window pair by pair.
+        const float h = J.h[0];
+        const float marg = (fabsf(bmin[0]) + fabsf(bmax[0]) + fabsf(J.mn[0]) + h) * 1.0e-5f;
+        const uint32_t ilo = box_of(bmax[0] - h + marg, J.mn[0], J.bs[0], J.nb[0]) + 1u;
+        const uint32_t ihi1 = box_of(bmin[0] + h - marg, J.mn[0], J.bs[0], J.nb[0]);
+        if (ilo < ihi1) {
+          m_lo = ilo;
+          m_n = ihi1 - ilo;
+        }
+      }
+    }
     uint32_t n1 = (D >= 2) ? (hi[1] - lo[1] + 1) : 1u;
     uint32_t n2 = (D == 3) ? (hi[2] - lo[2] + 1) : 1u;
     n_ranges = n1 * n2;
     if (n_ranges > (uint32_t)kMaxRanges) n_ranges = kMaxRanges + 1;  // signals the slow path
-    if (n_ranges <= (uint32_t)kMaxRanges && (uint32_t)l < n_ranges) {
+    if (m_n != 0) {
+      // two pair-by-pair ranges around the moment boxes (the job has a table)
+      n_ranges = 2;
+      if (l < 2) {
+        const uint32_t* st = cstart + J.cbase;
+        my_lo = st[l == 0 ? lo[0] : m_lo + m_n];
+        my_len = st[l == 0 ? m_lo : hi[0] + 1] - my_lo;
+      }
+    } else if (n_ranges <= (uint32_t)kMaxRanges && (uint32_t)l < n_ranges) {
       uint32_t r1 = (D >= 2) ? lo[1] + (uint32_t)l % n1 : 0u;
       uint32_t r2 = (D == 3) ? lo[2] + (uint32_t)l / n1 : 0u;
       uint64_t row = (uint64_t)r2 * J.nb[1] + r1;
@@ -1192,6 +1324,14 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
   };
   float nxt[3];
   load_cand(0u, nxt);
+  if constexpr (D == 1 && !RAW) {
+    if (m_n != 0) {
+      moment_boxes<Q>(J, mom, m_lo, m_n, xq, S, C, L);
+      mstat[0] = __builtin_amdgcn_readlane(my_lo, 1) -
+                 (__builtin_amdgcn_readlane(my_lo, 0) + __builtin_amdgcn_readlane(my_len, 0));
+      mstat[1] = m_n;
+    }
+  }
   for (uint32_t base = 0; base < total; base += kWave) chunk(base, nxt, kWave);
   wave_sync();
 
@@ -1340,22 +1480,31 @@ __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __r
 // Queries per lane chosen per tile: the smallest Q <= QMAX whose 64*Q lane
 // slots hold the tile (Q = 4 steps down through 3, 2 and 1), so partly
 // filled tiles do not pay for idle slots.
+// pairs counts every candidate pair of the tile, mpairs those summed from box
+// moments and mterms the (query, moment box) terms evaluated for them.
 template <int D, int QMAX, bool RAW, typename KeyT>
 __device__ __forceinline__ void run_tile(const DJob& J, const DTile& T, const float* __restrict__ X,
                                          const float* __restrict__ scoord, const KeyT* __restrict__ keys,
                                          const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ idx,
-                                         double* __restrict__ dens, WaveLds& L, uint64_t& pairs,
-                                         uint64_t& lane_slots) {
+                                         const float4* __restrict__ mom, double* __restrict__ dens, WaveLds& L,
+                                         uint64_t& pairs, uint64_t& lane_slots, uint64_t& mpairs,
+                                         uint64_t& mterms) {
   const uint32_t nq = T.p1 - T.p0;
   if constexpr (QMAX >= 2 && !RAW) {  // plain-path tiles are full except the last
     if (nq <= (uint32_t)(kWave * (QMAX - 1))) {
-      run_tile<D, QMAX - 1, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, pairs, lane_slots);
+      run_tile<D, QMAX - 1, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, dens, L, pairs, lane_slots,
+                                       mpairs, mterms);
       return;
     }
   }
-  const uint32_t total = process_tile<D, QMAX, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L);
-  pairs += (uint64_t)nq * total;
+  uint32_t mstat[2] = {0u, 0u};
+  const uint32_t total = process_tile<D, QMAX, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, dens, L, mstat);
+  pairs += (uint64_t)nq * (total + mstat[0]);
   lane_slots += (uint64_t)(kWave * QMAX) * ((total + kWave - 1) / kWave * kWave);
+  if constexpr (D == 1 && !RAW) {
+    mpairs += (uint64_t)nq * mstat[0];
+    mterms += (uint64_t)nq * mstat[1];
+  }
 }
 
 }  // namespace
@@ -1372,14 +1521,16 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
                                                     const KeyT* __restrict__ keys,
                                                     const uint32_t* __restrict__ cstart,
                                                     const uint32_t* __restrict__ idx,
+                                                    const float4* __restrict__ mom,
                                                     double* __restrict__ dens,
                                                     unsigned long long* __restrict__ pairs) {
   __shared__ WaveLds lds[kBlock / kWave];
   WaveLds& L = lds[threadIdx.x / kWave];
   const uint32_t n_tiles = *n_tiles_p;
   const int l = lane_id();
-  // executed pairs per dimension and lane slots, flushed once per wave
-  uint64_t acc[3] = {0, 0, 0}, slots = 0;
+  // candidate pairs per dimension, lane slots and the moment counts, flushed
+  // once per wave
+  uint64_t acc[3] = {0, 0, 0}, slots = 0, macc[2] = {0, 0};
   // Tiles are claimed from kClaimParts contiguous partitions of the tile
   // list, each with its own counter: a wave starts on the partition of its
   // workgroup's XCD (workgroups are dealt to the XCDs round robin) and moves
@@ -1414,15 +1565,23 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
           continue;
         }
       }
-      if (d == 1) run_tile<1, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[0], slots);
-      else if (d == 2) run_tile<2, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[1], slots);
-      else run_tile<3, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, dens, L, acc[2], slots);
+      if (d == 1)
+        run_tile<1, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, dens, L, acc[0], slots, macc[0],
+                                  macc[1]);
+      else if (d == 2)
+        run_tile<2, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, dens, L, acc[1], slots, macc[0],
+                                  macc[1]);
+      else
+        run_tile<3, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, dens, L, acc[2], slots, macc[0],
+                                  macc[1]);
     }
   }
   if (l == 0) {
     for (int e = 0; e < 3; ++e)
       if (acc[e]) atomicAdd(pairs + e, (unsigned long long)acc[e]);
     if (slots) atomicAdd(pairs + 3, (unsigned long long)slots);
+    for (int e = 0; e < 2; ++e)
+      if (macc[e]) atomicAdd(pairs + 4 + e, (unsigned long long)macc[e]);
   }
 }
 
@@ -1605,24 +1764,25 @@ template <int Q>
 static void launch_density_q(hipStream_t s, bool raw, bool wide, unsigned grid, const DJob* jobs,
                              const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
                              const float* scoord, const void* keys, const uint32_t* cstart,
-                             const uint32_t* idx, double* dens, unsigned long long* pairs) {
+                             const uint32_t* idx, const float4* mom, double* dens, unsigned long long* pairs) {
   if (raw)
     hipLaunchKernelGGL((k_density<Q, true, uint32_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
-                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, dens, pairs);
+                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, mom, dens, pairs);
   else if (wide)
     hipLaunchKernelGGL((k_density<Q, false, uint64_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
-                       counter, X, scoord, (const uint64_t*)keys, cstart, idx, dens, pairs);
+                       counter, X, scoord, (const uint64_t*)keys, cstart, idx, mom, dens, pairs);
   else
     hipLaunchKernelGGL((k_density<Q, false, uint32_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
-                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, dens, pairs);
+                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, mom, dens, pairs);
 }
 
 int launch_density(hipStream_t s, int q, bool raw, bool wide, unsigned grid, const DJob* jobs,
                    const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
                    const float* scoord, const void* keys, const uint32_t* cstart, const uint32_t* idx,
-                   double* dens, unsigned long long* pairs) {
-#define ENT_LD(QQ) \
-  launch_density_q<QQ>(s, raw, wide, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart, idx, dens, pairs)
+                   const float4* mom, double* dens, unsigned long long* pairs) {
+#define ENT_LD(QQ)                                                                                        \
+  launch_density_q<QQ>(s, raw, wide, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart, idx, mom, \
+                       dens, pairs)
   switch (q) {
     case 1: ENT_LD(1); break;
     case 2: ENT_LD(2); break;
@@ -1631,6 +1791,14 @@ int launch_density(hipStream_t s, int q, bool raw, bool wide, unsigned grid, con
   }
 #undef ENT_LD
   return 0;
+}
+
+void launch_box_moments(hipStream_t s, const DJob* jobs, const uint32_t* mlist, uint32_t max_waves,
+                        const float* scoord, const uint32_t* start, float4* mom) {
+  uint32_t blocks = (max_waves + (kBlock / kWave) - 1) / (kBlock / kWave);
+  if (blocks > 2048) blocks = 2048;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_box_moments, dim3(blocks), dim3(kBlock), 0, s, jobs, mlist, scoord, start, mom);
 }
 
 void launch_cell_starts(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,

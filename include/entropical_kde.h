@@ -77,7 +77,8 @@ int ent_ctx_destroy(ent_ctx* ctx);
 int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h);
 
 /* Statistics of the last density run: persistent-kernel time and whole
- * pipeline time (HIP events, ms), executed pair evaluations, tiles. */
+ * pipeline time (HIP events, ms), candidate pair evaluations (the pairs of
+ * every tile's candidate window), tiles. */
 int ent_ctx_last_stats(ent_ctx* ctx, double* density_ms, double* total_ms, uint64_t* pairs,
                        uint64_t* tiles);
 
@@ -95,11 +96,22 @@ int ent_ctx_last_sort(ent_ctx* ctx, uint32_t* out2);
  * reference counterpart. */
 int ent_ctx_set_graph(ent_ctx* ctx, int enable);
 
-/* Executed pair evaluations of the last density run, split by job dimension
+/* Candidate pair evaluations of the last density run, split by job dimension
  * (out[d-1] for d = 1, 2, 3), and out[3] = lane-slot evaluations issued
- * (64*Q lanes per tile times candidates rounded up to chunks; the ratio
- * sum(out[0..2]) / out[3] is the lane utilisation). */
+ * pair by pair (64*Q lanes per tile times candidates rounded up to chunks;
+ * the pairs evaluated pair by pair, sum(out[0..2]) less the moment pairs of
+ * ent_ctx_last_moments, over out[3] is the lane utilisation). */
 int ent_ctx_last_pairs(ent_ctx* ctx, uint64_t* out4);
+
+/* Box moments of the last density run: out[0] = candidate pairs of 1-D jobs
+ * (counted in ent_ctx_last_pairs) that were summed from box moments instead
+ * of pair by pair, out[1] = (query, box) moment terms evaluated for them.
+ * A box of a 1-D job that lies inside the support of every query of a tile
+ * contributes c + d (B - A d) per query (d = query - box origin; A, B, c from
+ * the box's point count and first two moments), which equals the sum of its
+ * pairs' kernel values.  ENT_BOX_MOMENTS=0 in the environment turns this off.
+ * No reference counterpart. */
+int ent_ctx_last_moments(ent_ctx* ctx, uint64_t* out2);
 
 /* Device memory helpers on the context's device and stream. */
 int ent_dev_alloc(ent_ctx* ctx, size_t bytes, void** ptr);

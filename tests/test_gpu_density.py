@@ -219,6 +219,35 @@ def test_packed_small_tiles(ctx, orc, monkeypatch, d):
     assert not np.array_equal(single, packed)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("h", [0.05, 0.2])
+def test_box_moments_1d(ctx, orc, monkeypatch, h):
+    """1-D boxes inside the support of every query of a tile are summed from
+    their moments (ENT_BOX_MOMENTS, on by default).  With moments and without
+    them the densities match FP64, the candidate pair count is the same, and
+    with moments on some of those pairs (and none with them off) come from
+    box moments."""
+    from entropical_amd import combined_densities
+
+    r = np.random.default_rng(41)
+    n = 60000
+    x = np.concatenate([r.standard_normal(n // 2), 0.7 * r.standard_t(4, size=n - n // 2)]).astype(np.float32)
+    x = r.permutation(x)[None, :]
+    ref = orc.density_fp64(x, n, [0], [h])
+    monkeypatch.setenv("ENT_BOX_MOMENTS", "0")
+    plain = combined_densities(x, n, [0], [h], ctx=ctx)
+    st0 = ctx.last_stats()
+    monkeypatch.setenv("ENT_BOX_MOMENTS", "1")
+    mom = combined_densities(x, n, [0], [h], ctx=ctx)
+    st1 = ctx.last_stats()
+    assert rel_err(plain, ref) <= RTOL
+    assert rel_err(mom, ref) <= RTOL
+    assert st0["moment_pairs"] == 0 and st0["moment_terms"] == 0
+    assert st1["pairs"] == st0["pairs"]
+    assert 0 < st1["moment_pairs"] < st1["pairs"]
+    assert 0 < st1["moment_terms"] < st1["moment_pairs"]
+
+
 # ---------------------------------------------------------- epa convolution
 def test_epa_convolution_matches_fp64(ctx, orc):
     from entropical_amd import epa_convolution
