@@ -16,13 +16,14 @@ d 1-3), entropy1d (N=100,000 three-component mixture, h=0.1).
 Printed JSON line: value = candidate pair evaluations per second
 (Gpair-evals/s) over the whole step: the (query, candidate) pairs of every
 tile's cell-list window, evaluated one by one or, for the boxes of a 1-D job
-that lie inside the support of every query of a tile, summed exactly from
-the box's moments (both counts are reported); the dense-equivalent rate, the plain
+and the cells of a 2-D job that lie inside the support of every query of a
+tile, summed exactly from their moments (both counts are reported); the
+dense-equivalent rate, the plain
 per-pair GPU path (the reference's per-call driver order, extrapolated from
 timed calls) and the host OpenMP baseline (oracle restatement of
 densities_omp.cpp on timed calls, extrapolated) are reported beside it.
 Roofline: FP32 VALU; achieved = algorithmic FLOPs (4d per pair evaluated one
-by one, 6 per (query, box) moment term) /
+by one, 6 per (query, box) and 25 per (query, cell) moment term) /
 duration of the persistent density kernel, measured with HIP events on the
 engine's stream.  The tile shape is chosen by the engine's autotuner during
 warmup (untimed).
@@ -40,8 +41,10 @@ import numpy as np
 PEAK_FP32_TFLOPS = 157.3
 FLOPS_PER_PAIR = (4, 8, 12)  # d = 1, 2, 3
 # (query, box) term of a 1-D box inside every query's support, which stands
-# for all of the box's pairs: d = x - r, t = B - A d, c + t d, accumulate
-MOMENT_TERM_FLOPS = 6
+# for all of the box's pairs: d = x - r, t = B - A d, c + t d, accumulate; and
+# (query, cell) term of a 2-D cell: 2 offsets, 2 x (scale, 1 - k d^2), 3
+# inner FMA pairs, 1 outer FMA pair, accumulate
+MOMENT_TERM_FLOPS = (6, 25)
 
 
 def log(msg: str) -> None:
@@ -90,16 +93,18 @@ def measured_traffic(workload: str):
 
 def evaluated_pairs(st) -> list:
     """Pairs evaluated one by one per job dimension: the candidate pairs less
-    the 1-D pairs summed from box moments."""
+    those summed from 1-D box and 2-D cell moments."""
     by_d = list(st["pairs_by_d"])
-    by_d[0] -= st["moment_pairs"]
+    for d, m in enumerate(st["moment_pairs_by_d"]):
+        by_d[d] -= m
     return by_d
 
 
 def roofline(st, kern_ms: float, workload: str) -> dict:
     """Algorithmic FLOP rate of k_density: 4d FLOPs per pair evaluated one by
-    one plus MOMENT_TERM_FLOPS per (query, box) moment term."""
-    flops = sum(p * f for p, f in zip(evaluated_pairs(st), FLOPS_PER_PAIR)) + st["moment_terms"] * MOMENT_TERM_FLOPS
+    one plus MOMENT_TERM_FLOPS per (query, box) or (query, cell) moment term."""
+    flops = sum(p * f for p, f in zip(evaluated_pairs(st), FLOPS_PER_PAIR)) + sum(
+        t * f for t, f in zip(st["moment_terms_by_d"], MOMENT_TERM_FLOPS))
     achieved = flops / (kern_ms * 1e-3) / 1e12
     return {
         "bound": "valu",
@@ -111,7 +116,8 @@ def roofline(st, kern_ms: float, workload: str) -> dict:
         "traffic": measured_traffic(workload),
         "kernel_ms": kern_ms,
         "flops_per_pair": "4d per pair evaluated one by one (d=1: 4, d=2: 8, d=3: 12); "
-                          "6 per (query, box) term of 1-D box moments",
+                          "6 per (query, box) term of 1-D box moments, 25 per (query, cell) term of 2-D "
+                          "cell moments",
     }
 
 
@@ -186,7 +192,7 @@ def run_mi(args) -> dict:
             "n_cols": n_cols, "n_rows": n_rows, "bandwidth": h, "method": "cell list",
             "tuning_q_m0_mr_group": list(tun),
         },
-        "pairs_candidate": pairs, "pairs_box_moments": st["moment_pairs"], "moment_terms": st["moment_terms"],
+        "pairs_candidate": pairs, "pairs_by_moments": st["moment_pairs"], "moment_terms": st["moment_terms"],
         "pairs_by_dim": st["pairs_by_d"],
         "lane_utilisation": util,
         "dense_equiv_gpair_per_s": 2080 * n_rows * n_rows / sec / 1e9,
@@ -245,7 +251,7 @@ def run_te(args) -> dict:
         "config": {"workload": "te_32series_100000frames_h0.2_tau1-2-5-10",
                    "jobs": f"{jobs_3d} 3-D + {jobs_2d} 2-D + {jobs_1d} 1-D densities, one persistent launch",
                    "tuning_q_m0_mr_group": list(tun)},
-        "pairs_candidate": pairs, "pairs_box_moments": st["moment_pairs"], "moment_terms": st["moment_terms"], "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
+        "pairs_candidate": pairs, "pairs_by_moments": st["moment_pairs"], "moment_terms": st["moment_terms"], "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
         "dense_equiv_gpair_per_s": dense / sec / 1e9,
         "roofline": roofline(st, kern_ms, "te"),
     }
@@ -301,7 +307,7 @@ def run_uneven(args) -> dict:
         "value": pairs / sec / 1e9, "unit": "Gpair-evals/s", "ms_per_step": 1e3 * sec,
         "config": {"workload": "uneven_500traj_2k-500k_d1-3", "points": int(sum(j["n"] for j in jobs)),
                    "tuning_q_m0_mr_group": list(tun)},
-        "pairs_candidate": pairs, "pairs_box_moments": st["moment_pairs"], "moment_terms": st["moment_terms"], "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
+        "pairs_candidate": pairs, "pairs_by_moments": st["moment_pairs"], "moment_terms": st["moment_terms"], "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
         "dense_equiv_gpair_per_s": dense / sec / 1e9,
         "roofline": roofline(st, kern_ms, "uneven"),
     }
@@ -325,7 +331,7 @@ def run_entropy1d(args) -> dict:
         "metric": "Gpair-evals/s (1-D KDE entropy, N=100k)", "value": pairs / sec / 1e9, "unit": "Gpair-evals/s",
         "ms_per_step": 1e3 * sec,
         "config": {"workload": "entropy1d_gmm3_100000_h0.1", "tuning_q_m0_mr_group": list(tun)},
-        "entropy": H, "pairs_candidate": pairs, "pairs_box_moments": st["moment_pairs"], "moment_terms": st["moment_terms"], "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
+        "entropy": H, "pairs_candidate": pairs, "pairs_by_moments": st["moment_pairs"], "moment_terms": st["moment_terms"], "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
         "dense_equiv_gpair_per_s": float(n) ** 2 / sec / 1e9, "roofline": roofline(st, kern_ms, "entropy1d"),
     }
     if not args.no_plain:

@@ -208,14 +208,16 @@ int ent_ctx_last_pairs(ent_ctx* ctx, uint64_t* out3) {  // out3: 4 entries
   return ENT_OK;
 }
 
-int ent_ctx_last_moments(ent_ctx* ctx, uint64_t* out2) {
+int ent_ctx_last_moments(ent_ctx* ctx, uint64_t* out4) {
   CAPI_TRY(check_ctx(ctx));
-  if (!out2) {
+  if (!out4) {
     set_error("null argument");
     return ENT_ERR_ARG;
   }
-  out2[0] = ctx->eng->last.moment_pairs;
-  out2[1] = ctx->eng->last.moment_terms;
+  for (int d = 0; d < 2; ++d) {
+    out4[d] = ctx->eng->last.moment_pairs[d];
+    out4[2 + d] = ctx->eng->last.moment_terms[d];
+  }
   return ENT_OK;
 }
 

@@ -59,7 +59,7 @@ Engine::~Engine() {
                     &d_keys[1], &d_idx[0], &d_idx[1],     &d_hist,   &d_scan_scratch, &d_scoord,
                     &d_tile_cnt, &d_tiles, &d_counter, &d_cellstart, &d_bjobs, &d_bchunks,
                     &d_bchunk_first, &d_bbase, &d_bkeys[0], &d_bkeys[1], &d_bidx[0], &d_bidx[1],
-                    &d_wjobs, &d_wchunks, &d_tchunks, &d_moments, &d_mlist};
+                    &d_wjobs, &d_wchunks, &d_tchunks, &d_moments, &d_mlist, &d_moments2, &d_mlist2};
   for (GraphCache* c : {&pre_graphs_, &dens_graphs_})
     for (auto& e : c->entries) (void)hipGraphExecDestroy(e.second);
   for (DevBuf* b : bufs)
@@ -290,28 +290,54 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   bool moments = true;
   if (const char* m = std::getenv("ENT_BOX_MOMENTS"))
     if (!std::strcmp(m, "0")) moments = false;
-  uint64_t mom_total = 0;
+  // 2-D jobs get a table of cells one row high and about h/4 wide in dim 0
+  // (m0/4 dim-0 boxes), when the job has a cell-start table and at most as
+  // many cells as points.
+  uint64_t mom_total = 0, mom2_total = 0;
   h_mlist.assign(1, 0u);
+  h_mlist2.assign(1, 0u);
   {
-    std::vector<uint32_t> mj;
-    uint32_t waves = 0;
+    std::vector<uint32_t> mj, mj2;
+    uint32_t waves = 0, blocks2 = 0;
     for (uint32_t j = 0; j < J; ++j) {
       DJob& D = h_jobs[j];
       D.flags &= ~kJobMoments;
       D.mbase = 0;
-      if (!moments || D.d != 1 || (D.flags & kJobTri) || D.cbase == kNoTable || D.nb[0] > D.n) continue;
-      D.flags |= kJobMoments;
-      D.mbase = mom_total;
-      mom_total += D.nb[0];
-      mj.push_back(j);
-      h_mlist.push_back(waves);
-      waves += (D.nb[0] + kMomentBoxesPerWave - 1) / kMomentBoxesPerWave;
+      D.mcf = 1;
+      D.mncb = D.nb[0];
+      if (!moments || D.cbase == kNoTable || (D.flags & kJobTri)) continue;
+      if (D.d == 1 && D.nb[0] <= D.n) {
+        D.flags |= kJobMoments;
+        D.mbase = mom_total;
+        mom_total += D.nb[0];
+        mj.push_back(j);
+        h_mlist.push_back(waves);
+        waves += (D.nb[0] + kMomentBoxesPerWave - 1) / kMomentBoxesPerWave;
+      } else if (D.d == 2) {
+        const uint32_t cf = (uint32_t)std::max(1, tuning.m0 / 4);
+        const uint32_t ncb = (D.nb[0] + cf - 1) / cf;
+        const uint64_t cells = (uint64_t)ncb * D.nb[1];
+        if (cells > D.n) continue;
+        D.flags |= kJobMoments;
+        D.mbase = mom2_total;
+        D.mcf = cf;
+        D.mncb = ncb;
+        mom2_total += cells;
+        mj2.push_back(j);
+        h_mlist2.push_back(blocks2);
+        blocks2 += (uint32_t)((cells + kBlock - 1) / kBlock);
+      }
     }
     h_mlist[0] = (uint32_t)mj.size();
     h_mlist.push_back(waves);
     h_mlist.insert(h_mlist.end(), mj.begin(), mj.end());
+    h_mlist2[0] = (uint32_t)mj2.size();
+    h_mlist2.push_back(blocks2);
+    h_mlist2.insert(h_mlist2.end(), mj2.begin(), mj2.end());
   }
   const bool has_mom = mom_total != 0;
+  const bool has_mom2 = mom2_total != 0;
+  const uint32_t mom2_blocks = (uint32_t)std::min<uint64_t>(P / kBlock + J, 1u << 30);
   // grid bound of k_box_moments from the plan's shape: the listed boxes are at
   // most the points
   const uint32_t mom_waves = (uint32_t)std::min<uint64_t>(P / kMomentBoxesPerWave + J, 1u << 30);
@@ -442,8 +468,9 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
                                          sizeof(uint64_t) * J
                                    : 0;
   const size_t mlist_bytes = sizeof(uint32_t) * h_mlist.size();
+  const size_t mlist2_bytes = sizeof(uint32_t) * h_mlist2.size();
   ENT_TRY(ensure_host(sizeof(DJob) * J + base_bytes + (upload_tchunks ? sizeof(uint2) * NC : 0) + mlist_bytes +
-                      64));
+                      mlist2_bytes + 80));
   {
     char* p = (char*)host_stage_;
     std::memcpy(p, h_jobs.data(), sizeof(DJob) * J);
@@ -454,6 +481,12 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
       std::memcpy(p, h_mlist.data(), mlist_bytes);
       ENT_HIP(hipMemcpyAsync(d_mlist.p, p, mlist_bytes, hipMemcpyHostToDevice, stream_));
       p += (mlist_bytes + 7) & ~(size_t)7;
+    }
+    if (has_mom2) {
+      ENT_TRY(ensure(d_mlist2, mlist2_bytes));
+      std::memcpy(p, h_mlist2.data(), mlist2_bytes);
+      ENT_HIP(hipMemcpyAsync(d_mlist2.p, p, mlist2_bytes, hipMemcpyHostToDevice, stream_));
+      p += (mlist2_bytes + 7) & ~(size_t)7;
     }
     if (upload_tchunks) {
       std::memcpy(p, h_tchunks.data(), sizeof(uint2) * NC);
@@ -496,6 +529,7 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   ENT_TRY(ensure(d_counter, sizeof(uint32_t) * kCounterWords));
   ENT_TRY(ensure(d_cellstart, sizeof(uint32_t) * std::max<uint64_t>(table_total, 1)));
   if (has_mom) ENT_TRY(ensure(d_moments, sizeof(float4) * mom_total));
+  if (has_mom2) ENT_TRY(ensure(d_moments2, sizeof(float) * 9 * mom2_total));
   if (shared) {
     for (int b = 0; b < 2; ++b) {
       ENT_TRY(ensure(d_bkeys[b], key_bytes * PB));
@@ -553,6 +587,9 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     if (has_mom)
       launch_box_moments(stream_, dj, (const uint32_t*)d_mlist.p, mom_waves, scoord, (const uint32_t*)d_cellstart.p,
                          (float4*)d_moments.p);
+    if (has_mom2)
+      launch_cell_moments2(stream_, dj, (const uint32_t*)d_mlist2.p, mom2_blocks, scoord,
+                           (const uint32_t*)d_cellstart.p, (float*)d_moments2.p);
     const uint2* tch = (const uint2*)d_tchunks.p;
     launch_tile_count(stream_, wide, dj, tch, NC, keys[cur], tq, tcnt);
     launch_exclusive_scan(stream_, tcnt, tcnt, NC, scratch);
@@ -568,7 +605,7 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   PlanKey key = plan_key(X, dens, J, NC, P, npass, false);
   key.NCB = shared ? NCB : 0;
   key.PB = shared ? PB : 0;
-  key.variant = (shared ? 1 : 0) | (staged ? 2 : 0) | (pack ? 4 : 0) | (npass0 << 3) | (has_mom ? 256 : 0);
+  key.variant = (shared ? 1 : 0) | (staged ? 2 : 0) | (pack ? 4 : 0) | (npass0 << 3) | (has_mom ? 256 : 0) | (has_mom2 ? 512 : 0);
   ENT_TRY(run_captured(pre_graphs_, key, [&] { issue_pre(true); }));
   return launch_density_common(false, X, dens, P);
 }
@@ -690,7 +727,8 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
                         n_tiles, counter, X, raw ? nullptr : (const float*)d_scoord.p,
                         raw ? nullptr : d_keys[sorted_buf].p, raw ? nullptr : (const uint32_t*)d_cellstart.p,
                         raw ? nullptr : (const uint32_t*)d_idx[sorted_buf].p,
-                        raw ? nullptr : (const float4*)d_moments.p, dens, (unsigned long long*)(counter + 2));
+                        raw ? nullptr : (const float4*)d_moments.p, raw ? nullptr : (const float*)d_moments2.p,
+                        dens, (unsigned long long*)(counter + 2));
   };
   ENT_HIP(hipEventRecord(ev_[1], stream_));
   if (raw) {
@@ -708,19 +746,21 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
   }
   ENT_HIP(hipEventRecord(ev_[2], stream_));
   // stats readback
-  ENT_TRY(ensure_host(64));
+  ENT_TRY(ensure_host(80));
   uint32_t* hc = (uint32_t*)host_stage_;
-  ENT_HIP(hipMemcpyAsync(hc, counter, 56, hipMemcpyDeviceToHost, stream_));
-  ENT_HIP(hipMemcpyAsync(hc + 14, n_tiles, 4, hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipMemcpyAsync(hc, counter, 72, hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipMemcpyAsync(hc + 18, n_tiles, 4, hipMemcpyDeviceToHost, stream_));
   ENT_HIP(hipStreamSynchronize(stream_));
-  uint64_t pd[6];
-  std::memcpy(pd, hc + 2, 48);
+  uint64_t pd[8];
+  std::memcpy(pd, hc + 2, 64);
   last.lane_pairs = pd[3];
   last.pairs = pd[0] + pd[1] + pd[2];
   for (int d = 0; d < 3; ++d) last.pairs_by_d[d] = pd[d];
-  last.moment_pairs = pd[4];
-  last.moment_terms = pd[5];
-  last.tiles = hc[14];
+  for (int d = 0; d < 2; ++d) {
+    last.moment_pairs[d] = pd[4 + d];
+    last.moment_terms[d] = pd[6 + d];
+  }
+  last.tiles = hc[18];
   float ms = 0.f;
   ENT_HIP(hipEventElapsedTime(&ms, ev_[1], ev_[2]));
   last.density_ms = ms;

@@ -66,8 +66,9 @@ struct RunStats {
   uint64_t pairs = 0;        // executed (query, candidate) evaluations
   uint64_t pairs_by_d[3] = {0, 0, 0};  // the same, per job dimension
   uint64_t lane_pairs = 0;   // lane-slot evaluations issued (incl. idle lanes / padding)
-  uint64_t moment_pairs = 0;  // candidate pairs (within pairs) summed from 1-D box moments
-  uint64_t moment_terms = 0;  // (query, moment box) terms evaluated for them
+  uint64_t moment_pairs[2] = {0, 0};  // candidate pairs (within pairs_by_d) of 1-D / 2-D jobs
+                                      // summed from box / cell moments
+  uint64_t moment_terms[2] = {0, 0};  // (query, box) terms of 1-D jobs, (query, cell) of 2-D jobs
   uint64_t tiles = 0;
   uint32_t radix_passes = 0;  // base passes + job passes
   uint32_t sort_bases = 0;    // shared dim-0 sorts (0: one sort per job by full key)
@@ -145,7 +146,8 @@ class Engine {
   DevBuf d_wjobs, d_wchunks;
   // chunk list in tile order (jobs with the most candidates per query first)
   DevBuf d_tchunks;
-  DevBuf d_moments, d_mlist;  // box-moment table of the 1-D jobs and its wave list
+  DevBuf d_moments, d_mlist;    // box-moment table of the 1-D jobs and its wave list
+  DevBuf d_moments2, d_mlist2;  // cell-moment table of the 2-D jobs and its block list
   // host copies
   std::vector<DJob> h_jobs;
   std::vector<uint2> h_chunks;
@@ -161,6 +163,7 @@ class Engine {
   std::vector<uint32_t> h_tord;      // job order of the last upload
   std::vector<uint32_t> h_tsizes;    // job sizes of the last upload
   std::vector<uint32_t> h_mlist;     // box-moment wave list (k_box_moments)
+  std::vector<uint32_t> h_mlist2;    // cell-moment block list (k_cell_moments2)
   uint64_t total_points = 0;
   int sorted_buf = 0;  // which of d_keys/d_idx holds the sorted result
   bool keys_wide = false;  // cell keys stored as uint64 (else uint32)

@@ -31,9 +31,10 @@ constexpr uint32_t kJobTri = 2u;    // 1-D triangular kernel max(0, 1-|u|): the
                                     // Epanechnikov self-convolution of epa_convolution
 constexpr uint32_t kJobBase2 = 8u;  // two-stage sort of a 3-D job: its base is sorted by
                                     // dims 0 and 1, its own passes are over dim 2
-constexpr uint32_t kJobMoments = 16u;  // 1-D job with a box-moment table: the boxes that lie
-                                       // inside every query's support are summed from
-                                       // their moments instead of pair by pair
+constexpr uint32_t kJobMoments = 16u;  // 1-D or 2-D job with a moment table: the boxes (1-D)
+                                       // or cells (2-D) that lie inside every query's
+                                       // support are summed from their moments instead
+                                       // of pair by pair
 
 // Device job descriptor.  Points of job j, dim e, index k in [0, n):
 //   X[off[e] + k].
@@ -55,6 +56,8 @@ struct DJob {
   uint64_t cells;    // prod_e nb[e]
   uint64_t cbase;    // first entry of this job's cell-start table, or kNoTable
   uint64_t mbase;    // first entry of this job's box-moment table (kJobMoments)
+  uint32_t mcf;      // 2-D moment cells: dim-0 boxes per cell
+  uint32_t mncb;     // 2-D moment cells per row
 };
 
 // Jobs whose cell count exceeds this multiple of their points (sparse grids,

@@ -798,6 +798,70 @@ __global__ __launch_bounds__(kBlock) void k_box_moments(const DJob* __restrict__
   }
 }
 
+// Moment table of the 2-D jobs flagged kJobMoments, one thread per cell.
+// Cell (row, cb) of job J covers dim-0 boxes [cb mcf, cb mcf + mcf) of one
+// row; with origin (r0, r1) = (fma(cb mcf, bs0, mn0), fma(row, bs1, mn1)) and
+// offsets (e_j, g_j) = (x_j - r0, y_j - r1) of its points, it stores the 9
+// floats N[3p + q] = s0_p s1_q sum_j e_j^p g_j^q (p, q = 0..2; sums in FP64)
+// with s_e = (1, 2/h_e^2, -1/h_e^2), so that for a query (x, y) with
+// d = (x - r0, y - r1) whose support holds the whole cell,
+//   sum_j (1 - ((x - x_j)/h0)^2)(1 - ((y - y_j)/h1)^2)
+//     = sum_q b_q sum_p a_p N[3p + q],  a = (1 - d0^2/h0^2, d0, 1), b = (1 - d1^2/h1^2, d1, 1).
+// mlist2: [0] = M jobs, [1 .. M+1] = first block of each job (prefix, kBlock
+// cells per block), then the M job indices.  The grid strides over the blocks.
+__global__ __launch_bounds__(kBlock) void k_cell_moments2(const DJob* __restrict__ jobs,
+                                                          const uint32_t* __restrict__ mlist,
+                                                          const float* __restrict__ scoord,
+                                                          const uint32_t* __restrict__ start,
+                                                          float* __restrict__ mom) {
+  const uint32_t M = mlist[0];
+  const uint32_t* bfirst = mlist + 1;
+  const uint32_t* mjob = mlist + 2 + M;
+  const uint32_t n_blocks = bfirst[M];
+  for (uint32_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    uint32_t lo = 0, hi = M;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (bfirst[mid] <= blk) lo = mid;
+      else hi = mid;
+    }
+    const DJob& J = jobs[mjob[lo]];
+    const uint32_t cell = (blk - bfirst[lo]) * kBlock + threadIdx.x;
+    const uint32_t ncb = J.mncb, nb0 = J.nb[0];
+    if (cell >= J.nb[1] * ncb) continue;
+    const uint32_t row = cell / ncb;
+    const uint32_t cb = cell - row * ncb;
+    const uint32_t f0 = cb * J.mcf;
+    const uint32_t f1 = min(f0 + J.mcf, nb0);
+    const uint32_t* st = start + J.cbase + (uint64_t)row * nb0;
+    const uint32_t a = st[f0], e = st[f1];
+    const double r0 = (double)fmaf((float)f0, J.bs[0], J.mn[0]);
+    const double r1 = (double)fmaf((float)row, J.bs[1], J.mn[1]);
+    const float* x = scoord + 3 * J.pbase;
+    const float* y = x + J.n;
+    double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t p = a; p < e; ++p) {
+      const double ex = (double)x[p] - r0, gy = (double)y[p] - r1;
+      const double g2 = gy * gy;
+      const double pw[3] = {1.0, ex, ex * ex};
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        m[3 * i] += pw[i];
+        m[3 * i + 1] = fma(pw[i], gy, m[3 * i + 1]);
+        m[3 * i + 2] = fma(pw[i], g2, m[3 * i + 2]);
+      }
+    }
+    const double k0 = (double)J.hinv[0] * (double)J.hinv[0];
+    const double k1 = (double)J.hinv[1] * (double)J.hinv[1];
+    const double s0[3] = {1.0, 2.0 * k0, -k0}, s1[3] = {1.0, 2.0 * k1, -k1};
+    float* out = mom + (J.mbase + cell) * 9;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) out[3 * i + k] = (float)(m[3 * i + k] * s0[i] * s1[k]);
+  }
+}
+
 // ------------------------------------------------------------------ tiles
 // A tile starts at sorted position p when p == 0, p % tq == 0, or the
 // (row, dim-0 group) of p differs from that of p-1.  Tiles never cross rows,
@@ -977,6 +1041,7 @@ __device__ __forceinline__ uint32_t lower_bound_key(const KeyT* a, uint32_t n, u
 struct WaveLds {
   float4 cand[kWave];        // staged candidates (local coordinates)
   uint2 rng[kWave];          // packed tiles: candidate ranges (first flattened index, first position)
+  float4 cell[2][kWave];     // 2-D moment cells (with cand): 9 moments and the cell origin
 };
 
 // Conservative box range [lo, hi] covering [a - h, b + h] along one dim.
@@ -1122,6 +1187,63 @@ __device__ __forceinline__ void moment_boxes(const DJob& J, const float4* __rest
   }
 }
 
+// Moment cells of a 2-D tile (rows [r0, r0 + nr) x cells [c0, c0 + nc) of
+// each row) added into the lanes' queries: 64 cells at a time, lane k stages
+// cell k's moments N and origin in LDS, then every lane adds
+// sum_q b_q sum_p a_p N[3p + q] per cell (padding cells are zero), with a
+// compensated merge every 8 cells.
+template <int Q>
+__device__ __forceinline__ void cell_moments(const DJob& J, const float* __restrict__ mom2, uint32_t r0,
+                                             uint32_t nr, uint32_t c0, uint32_t nc, const float (&xq)[3][Q],
+                                             float (&S)[Q], float (&C)[Q], WaveLds& L) {
+  const int l = lane_id();
+  const uint32_t cells = nr * nc;
+  const float k0 = J.hinv[0] * J.hinv[0], k1 = J.hinv[1] * J.hinv[1];
+  for (uint32_t base = 0; base < cells; base += kWave) {
+    const uint32_t cnt = min(cells - base, (uint32_t)kWave);
+    float N[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float o0 = 0.f, o1 = 0.f;
+    if ((uint32_t)l < cnt) {
+      const uint32_t k = base + (uint32_t)l;
+      const uint32_t ri = k / nc;
+      const uint32_t row = r0 + ri, cb = c0 + (k - ri * nc);
+      const float* src = mom2 + (J.mbase + (uint64_t)row * J.mncb + cb) * 9;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) N[i] = src[i];
+      o0 = fmaf((float)(cb * J.mcf), J.bs[0], J.mn[0]);
+      o1 = fmaf((float)row, J.bs[1], J.mn[1]);
+    }
+    wave_sync();
+    L.cand[l] = make_float4(N[0], N[1], N[2], N[3]);
+    L.cell[0][l] = make_float4(N[4], N[5], N[6], N[7]);
+    L.cell[1][l] = make_float4(N[8], o0, o1, 0.f);
+    wave_sync();
+    const uint32_t cnt8 = (cnt + 7u) & ~7u;
+    for (uint32_t k = 0; k < cnt8; k += 8) {
+      float part[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) part[q] = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        // a = (N00, N01, N02, N10), b = (N11, N12, N20, N21), c = (N22, r0, r1, -)
+        const float4 a = L.cand[k + i], b = L.cell[0][k + i], c = L.cell[1][k + i];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const float d0 = xq[0][q] - c.y, d1 = xq[1][q] - c.z;
+          const float a0 = __builtin_fmaf(-(k0 * d0), d0, 1.0f);
+          const float b0 = __builtin_fmaf(-(k1 * d1), d1, 1.0f);
+          const float w0 = __builtin_fmaf(a0, a.x, __builtin_fmaf(d0, a.w, b.z));
+          const float w1 = __builtin_fmaf(a0, a.y, __builtin_fmaf(d0, b.x, b.w));
+          const float w2 = __builtin_fmaf(a0, a.z, __builtin_fmaf(d0, b.y, c.x));
+          part[q] += __builtin_fmaf(b0, w0, __builtin_fmaf(d1, w1, w2));
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < Q; ++q) kahan_add(S[q], C[q], part[q]);
+    }
+  }
+}
+
 // Densities of one tile's queries; returns the tile's pair-by-pair candidate
 // count (wave-uniform).  mstat (1-D jobs with a moment table): candidates
 // summed from box moments and the number of those boxes.
@@ -1129,8 +1251,8 @@ template <int D, int Q, bool RAW, typename KeyT>
 __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __restrict__ X,
                                  const float* __restrict__ scoord, const KeyT* __restrict__ keys,
                                  const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ idx,
-                                 const float4* __restrict__ mom, double* __restrict__ dens, WaveLds& L,
-                                 uint32_t (&mstat)[2]) {
+                                 const float4* __restrict__ mom, const float* __restrict__ mom2,
+                                 double* __restrict__ dens, WaveLds& L, uint32_t (&mstat)[2]) {
   const int l = lane_id();
   const uint32_t n = J.n;
   const float* src[3];
@@ -1158,7 +1280,8 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
   float o[3] = {0.f, 0.f, 0.f};
   uint32_t n_ranges = 1;
   uint32_t my_lo = 0, my_len = 0;
-  uint32_t m_lo = 0, m_n = 0;  // boxes summed from the moment table
+  uint32_t m_lo = 0, m_n = 0;  // 1-D: boxes summed from the moment table
+  uint32_t c_r0 = 0, c_nr = 0, c_c0 = 0, c_nc = 0, c_pts = 0;  // 2-D: moment cells and their points
   if constexpr (RAW) {
     if (l == 0) { my_lo = 0; my_len = n; }
   } else {
@@ -1226,6 +1349,54 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
       }
       my_lo = a;
       my_len = b - a;
+    }
+    if constexpr (D == 2) {
+      if ((J.flags & kJobMoments) && n_ranges <= (uint32_t)kMaxRanges / 2) {
+        // Cells (row, cb) of rows [jlo, jhi1) and dim-0 cells [clo, chi1)
+        // hold only points inside the support of every query (as for 1-D
+        // boxes, per dim).  Used when they hold at least 4 points per cell
+        // on average; the interior rows' pair-by-pair candidates become two
+        // ranges, left and right of the cells (the right ones on lanes
+        // n1 .. n1 + nir - 1).
+        const uint32_t cf = J.mcf;
+        const float h0 = J.h[0], h1 = J.h[1];
+        const float mg0 = (fabsf(bmin[0]) + fabsf(bmax[0]) + fabsf(J.mn[0]) + h0) * 1.0e-5f;
+        const float mg1 = (fabsf(bmin[1]) + fabsf(bmax[1]) + fabsf(J.mn[1]) + h1) * 1.0e-5f;
+        const uint32_t ilo = box_of(bmax[0] - h0 + mg0, J.mn[0], J.bs[0], J.nb[0]) + 1u;
+        const uint32_t ihi1 = box_of(bmin[0] + h0 - mg0, J.mn[0], J.bs[0], J.nb[0]);
+        const uint32_t clo = (ilo + cf - 1u) / cf, chi1 = ihi1 / cf;
+        const uint32_t jlo = max(box_of(bmax[1] - h1 + mg1, J.mn[1], J.bs[1], J.nb[1]) + 1u, lo[1]);
+        const uint32_t jhi1 = min(box_of(bmin[1] + h1 - mg1, J.mn[1], J.bs[1], J.nb[1]), hi[1] + 1u);
+        if (clo < chi1 && jlo < jhi1) {
+          const uint32_t nir = jhi1 - jlo, nc = chi1 - clo;
+          const uint32_t r = lo[1] + (uint32_t)l;
+          const bool inr = (uint32_t)l < n_ranges && r >= jlo && r < jhi1;
+          uint32_t m1 = 0, m2 = 0;
+          if (inr) {
+            const uint32_t* st = cstart + J.cbase + (uint64_t)r * J.nb[0];
+            m1 = st[clo * cf];
+            m2 = st[chi1 * cf];
+          }
+          const uint32_t pts = __builtin_amdgcn_readlane(wave_incl_scan(m2 - m1), kWave - 1);
+          if (pts >= 4u * nir * nc) {
+            const uint32_t end = my_lo + my_len;
+            const int src = ((int)(jlo - lo[1]) + l - (int)n_ranges) & (kWave - 1);
+            const uint32_t s_m2 = __shfl(m2, src);
+            const uint32_t s_end = __shfl(end, src);
+            if (inr) my_len = m1 - my_lo;
+            if ((uint32_t)l >= n_ranges && (uint32_t)l < n_ranges + nir) {
+              my_lo = s_m2;
+              my_len = s_end - s_m2;
+            }
+            n_ranges += nir;
+            c_r0 = jlo;
+            c_nr = nir;
+            c_c0 = clo;
+            c_nc = nc;
+            c_pts = pts;
+          }
+        }
+      }
     }
     if (n_ranges > (uint32_t)kMaxRanges) {
       // More candidate rows than lanes: fall back to one range spanning every
@@ -1330,6 +1501,13 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
       mstat[0] = __builtin_amdgcn_readlane(my_lo, 1) -
                  (__builtin_amdgcn_readlane(my_lo, 0) + __builtin_amdgcn_readlane(my_len, 0));
       mstat[1] = m_n;
+    }
+  }
+  if constexpr (D == 2 && !RAW) {
+    if (c_nr != 0) {
+      cell_moments<Q>(J, mom2, c_r0, c_nr, c_c0, c_nc, xq, S, C, L);
+      mstat[0] = c_pts;
+      mstat[1] = c_nr * c_nc;
     }
   }
   for (uint32_t base = 0; base < total; base += kWave) chunk(base, nxt, kWave);
@@ -1480,30 +1658,32 @@ __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __r
 // Queries per lane chosen per tile: the smallest Q <= QMAX whose 64*Q lane
 // slots hold the tile (Q = 4 steps down through 3, 2 and 1), so partly
 // filled tiles do not pay for idle slots.
-// pairs counts every candidate pair of the tile, mpairs those summed from box
-// moments and mterms the (query, moment box) terms evaluated for them.
+// pairs counts every candidate pair of the tile; macc[d-1] those summed from
+// moments in d-dim jobs (d = 1, 2), macc[d+1] the (query, box) / (query,
+// cell) moment terms evaluated for them.
 template <int D, int QMAX, bool RAW, typename KeyT>
 __device__ __forceinline__ void run_tile(const DJob& J, const DTile& T, const float* __restrict__ X,
                                          const float* __restrict__ scoord, const KeyT* __restrict__ keys,
                                          const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ idx,
-                                         const float4* __restrict__ mom, double* __restrict__ dens, WaveLds& L,
-                                         uint64_t& pairs, uint64_t& lane_slots, uint64_t& mpairs,
-                                         uint64_t& mterms) {
+                                         const float4* __restrict__ mom, const float* __restrict__ mom2,
+                                         double* __restrict__ dens, WaveLds& L, uint64_t& pairs,
+                                         uint64_t& lane_slots, uint64_t (&macc)[4]) {
   const uint32_t nq = T.p1 - T.p0;
   if constexpr (QMAX >= 2 && !RAW) {  // plain-path tiles are full except the last
     if (nq <= (uint32_t)(kWave * (QMAX - 1))) {
-      run_tile<D, QMAX - 1, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, dens, L, pairs, lane_slots,
-                                       mpairs, mterms);
+      run_tile<D, QMAX - 1, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, mom2, dens, L, pairs, lane_slots,
+                                       macc);
       return;
     }
   }
   uint32_t mstat[2] = {0u, 0u};
-  const uint32_t total = process_tile<D, QMAX, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, dens, L, mstat);
+  const uint32_t total =
+      process_tile<D, QMAX, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, mom2, dens, L, mstat);
   pairs += (uint64_t)nq * (total + mstat[0]);
   lane_slots += (uint64_t)(kWave * QMAX) * ((total + kWave - 1) / kWave * kWave);
-  if constexpr (D == 1 && !RAW) {
-    mpairs += (uint64_t)nq * mstat[0];
-    mterms += (uint64_t)nq * mstat[1];
+  if constexpr (D <= 2 && !RAW) {
+    macc[D - 1] += (uint64_t)nq * mstat[0];
+    macc[D + 1] += (uint64_t)nq * mstat[1];
   }
 }
 
@@ -1522,6 +1702,7 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
                                                     const uint32_t* __restrict__ cstart,
                                                     const uint32_t* __restrict__ idx,
                                                     const float4* __restrict__ mom,
+                                                    const float* __restrict__ mom2,
                                                     double* __restrict__ dens,
                                                     unsigned long long* __restrict__ pairs) {
   __shared__ WaveLds lds[kBlock / kWave];
@@ -1530,7 +1711,7 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
   const int l = lane_id();
   // candidate pairs per dimension, lane slots and the moment counts, flushed
   // once per wave
-  uint64_t acc[3] = {0, 0, 0}, slots = 0, macc[2] = {0, 0};
+  uint64_t acc[3] = {0, 0, 0}, slots = 0, macc[4] = {0, 0, 0, 0};
   // Tiles are claimed from kClaimParts contiguous partitions of the tile
   // list, each with its own counter: a wave starts on the partition of its
   // workgroup's XCD (workgroups are dealt to the XCDs round robin) and moves
@@ -1566,21 +1747,18 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
         }
       }
       if (d == 1)
-        run_tile<1, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, dens, L, acc[0], slots, macc[0],
-                                  macc[1]);
+        run_tile<1, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, mom2, dens, L, acc[0], slots, macc);
       else if (d == 2)
-        run_tile<2, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, dens, L, acc[1], slots, macc[0],
-                                  macc[1]);
+        run_tile<2, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, mom2, dens, L, acc[1], slots, macc);
       else
-        run_tile<3, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, dens, L, acc[2], slots, macc[0],
-                                  macc[1]);
+        run_tile<3, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, mom2, dens, L, acc[2], slots, macc);
     }
   }
   if (l == 0) {
     for (int e = 0; e < 3; ++e)
       if (acc[e]) atomicAdd(pairs + e, (unsigned long long)acc[e]);
     if (slots) atomicAdd(pairs + 3, (unsigned long long)slots);
-    for (int e = 0; e < 2; ++e)
+    for (int e = 0; e < 4; ++e)
       if (macc[e]) atomicAdd(pairs + 4 + e, (unsigned long long)macc[e]);
   }
 }
@@ -1764,25 +1942,26 @@ template <int Q>
 static void launch_density_q(hipStream_t s, bool raw, bool wide, unsigned grid, const DJob* jobs,
                              const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
                              const float* scoord, const void* keys, const uint32_t* cstart,
-                             const uint32_t* idx, const float4* mom, double* dens, unsigned long long* pairs) {
+                             const uint32_t* idx, const float4* mom, const float* mom2, double* dens,
+                             unsigned long long* pairs) {
   if (raw)
     hipLaunchKernelGGL((k_density<Q, true, uint32_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
-                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, mom, dens, pairs);
+                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, mom, mom2, dens, pairs);
   else if (wide)
     hipLaunchKernelGGL((k_density<Q, false, uint64_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
-                       counter, X, scoord, (const uint64_t*)keys, cstart, idx, mom, dens, pairs);
+                       counter, X, scoord, (const uint64_t*)keys, cstart, idx, mom, mom2, dens, pairs);
   else
     hipLaunchKernelGGL((k_density<Q, false, uint32_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
-                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, mom, dens, pairs);
+                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, mom, mom2, dens, pairs);
 }
 
 int launch_density(hipStream_t s, int q, bool raw, bool wide, unsigned grid, const DJob* jobs,
                    const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
                    const float* scoord, const void* keys, const uint32_t* cstart, const uint32_t* idx,
-                   const float4* mom, double* dens, unsigned long long* pairs) {
+                   const float4* mom, const float* mom2, double* dens, unsigned long long* pairs) {
 #define ENT_LD(QQ)                                                                                        \
   launch_density_q<QQ>(s, raw, wide, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart, idx, mom, \
-                       dens, pairs)
+                       mom2, dens, pairs)
   switch (q) {
     case 1: ENT_LD(1); break;
     case 2: ENT_LD(2); break;
@@ -1799,6 +1978,14 @@ void launch_box_moments(hipStream_t s, const DJob* jobs, const uint32_t* mlist, 
   if (blocks > 2048) blocks = 2048;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_box_moments, dim3(blocks), dim3(kBlock), 0, s, jobs, mlist, scoord, start, mom);
+}
+
+void launch_cell_moments2(hipStream_t s, const DJob* jobs, const uint32_t* mlist, uint32_t max_blocks,
+                          const float* scoord, const uint32_t* start, float* mom) {
+  uint32_t blocks = max_blocks;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_cell_moments2, dim3(blocks), dim3(kBlock), 0, s, jobs, mlist, scoord, start, mom);
 }
 
 void launch_cell_starts(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,

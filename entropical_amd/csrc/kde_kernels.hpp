@@ -60,13 +60,18 @@ int density_occupancy(int q, bool raw, bool wide, int* blocks_per_cu);
 int launch_density(hipStream_t s, int q, bool raw, bool wide, unsigned grid, const DJob* jobs,
                    const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
                    const float* scoord, const void* keys, const uint32_t* cstart, const uint32_t* idx,
-                   const float4* mom, double* dens, unsigned long long* pairs);
+                   const float4* mom, const float* mom2, double* dens, unsigned long long* pairs);
 // Box-moment table of the kJobMoments jobs listed in mlist ([0] = M, [1..M+1]
 // = first wave per job, then the M job indices); max_waves bounds the list's
 // waves (sets the grid, which strides over them).
 void launch_box_moments(hipStream_t s, const DJob* jobs, const uint32_t* mlist, uint32_t max_waves,
                         const float* scoord, const uint32_t* start, float4* mom);
 constexpr uint32_t kMomentBoxesPerWave = 16;
+// Moment table of the 2-D kJobMoments jobs listed in mlist ([0] = M, [1..M+1]
+// = first block per job, kBlock cells per block, then the M job indices);
+// max_blocks bounds the list's blocks.
+void launch_cell_moments2(hipStream_t s, const DJob* jobs, const uint32_t* mlist, uint32_t max_blocks,
+                          const float* scoord, const uint32_t* start, float* mom);
 void launch_cell_starts(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
                         const void* keys, uint32_t* start);
 

@@ -103,15 +103,18 @@ int ent_ctx_set_graph(ent_ctx* ctx, int enable);
  * ent_ctx_last_moments, over out[3] is the lane utilisation). */
 int ent_ctx_last_pairs(ent_ctx* ctx, uint64_t* out4);
 
-/* Box moments of the last density run: out[0] = candidate pairs of 1-D jobs
- * (counted in ent_ctx_last_pairs) that were summed from box moments instead
- * of pair by pair, out[1] = (query, box) moment terms evaluated for them.
- * A box of a 1-D job that lies inside the support of every query of a tile
- * contributes c + d (B - A d) per query (d = query - box origin; A, B, c from
- * the box's point count and first two moments), which equals the sum of its
- * pairs' kernel values.  ENT_BOX_MOMENTS=0 in the environment turns this off.
- * No reference counterpart. */
-int ent_ctx_last_moments(ent_ctx* ctx, uint64_t* out2);
+/* Moment sums of the last density run: out[d-1] = candidate pairs of d-dim
+ * jobs (d = 1, 2; counted in ent_ctx_last_pairs) summed from box or cell
+ * moments instead of pair by pair, out[2] = (query, box) terms evaluated for
+ * them in 1-D jobs, out[3] = (query, cell) terms in 2-D jobs.  A 1-D box, or
+ * a 2-D cell (one row high,
+ * about h/4 wide), that lies inside the support of every query of a tile
+ * contributes a polynomial in the query's offset from its origin, whose
+ * coefficients are the box's point count and moments up to second order
+ * per dim: it equals the sum of the pairs' kernel values.
+ * ENT_BOX_MOMENTS=0 in the environment turns this off.  No reference
+ * counterpart. */
+int ent_ctx_last_moments(ent_ctx* ctx, uint64_t* out4);
 
 /* Device memory helpers on the context's device and stream. */
 int ent_dev_alloc(ent_ctx* ctx, size_t bytes, void** ptr);

@@ -248,6 +248,34 @@ def test_box_moments_1d(ctx, orc, monkeypatch, h):
     assert 0 < st1["moment_terms"] < st1["moment_pairs"]
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("h", [(0.1, 0.15), (0.3, 0.2)])
+def test_cell_moments_2d(ctx, orc, monkeypatch, h):
+    """2-D cells (one row high, about h/4 wide) inside the support of every
+    query of a dense tile are summed from their 9 moments.  With moments and
+    without them the densities match FP64 and the candidate pair count is
+    the same; with moments some 2-D pairs come from cell moments."""
+    from entropical_amd import combined_densities
+
+    r = np.random.default_rng(43)
+    n = 80000
+    a = r.standard_normal(n)
+    x = np.stack([a, 0.6 * a + 0.8 * r.standard_normal(n)]).astype(np.float32)
+    ref = orc.density_fp64(x, n, [0, 1], list(h))
+    monkeypatch.setenv("ENT_BOX_MOMENTS", "0")
+    plain = combined_densities(x, n, [0, 1], list(h), ctx=ctx)
+    st0 = ctx.last_stats()
+    monkeypatch.setenv("ENT_BOX_MOMENTS", "1")
+    mom = combined_densities(x, n, [0, 1], list(h), ctx=ctx)
+    st1 = ctx.last_stats()
+    assert rel_err(plain, ref) <= RTOL
+    assert rel_err(mom, ref) <= RTOL
+    assert st0["moment_pairs"] == 0
+    assert st1["pairs"] == st0["pairs"]
+    assert 0 < st1["moment_pairs_by_d"][1] < st1["pairs"]
+    assert 0 < st1["moment_terms_by_d"][1] < st1["moment_pairs_by_d"][1]
+
+
 # ---------------------------------------------------------- epa convolution
 def test_epa_convolution_matches_fp64(ctx, orc):
     from entropical_amd import epa_convolution
