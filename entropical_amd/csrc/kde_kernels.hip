@@ -840,8 +840,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_moments2(const DJob* __restrict
     const float* x = scoord + 3 * J.pbase;
     const float* y = x + J.n;
     double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t p = a; p < e; ++p) {
-      const double ex = (double)x[p] - r0, gy = (double)y[p] - r1;
+    auto add = [&](float xv, float yv) {
+      const double ex = (double)xv - r0, gy = (double)yv - r1;
       const double g2 = gy * gy;
       const double pw[3] = {1.0, ex, ex * ex};
 #pragma unroll
@@ -850,7 +850,22 @@ __global__ __launch_bounds__(kBlock) void k_cell_moments2(const DJob* __restrict
         m[3 * i + 1] = fma(pw[i], gy, m[3 * i + 1]);
         m[3 * i + 2] = fma(pw[i], g2, m[3 * i + 2]);
       }
+    };
+    // eight points per step: their loads are issued together, so a step
+    // waits for one memory latency rather than eight
+    constexpr uint32_t kStep = 8;
+    uint32_t p = a;
+    for (; p + kStep <= e; p += kStep) {
+      float xv[kStep], yv[kStep];
+#pragma unroll
+      for (uint32_t i = 0; i < kStep; ++i) {
+        xv[i] = x[p + i];
+        yv[i] = y[p + i];
+      }
+#pragma unroll
+      for (uint32_t i = 0; i < kStep; ++i) add(xv[i], yv[i]);
     }
+    for (; p < e; ++p) add(x[p], y[p]);
     const double k0 = (double)J.hinv[0] * (double)J.hinv[0];
     const double k1 = (double)J.hinv[1] * (double)J.hinv[1];
     const double s0[3] = {1.0, 2.0 * k0, -k0}, s1[3] = {1.0, 2.0 * k1, -k1};
