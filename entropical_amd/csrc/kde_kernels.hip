@@ -1186,7 +1186,7 @@ __device__ __forceinline__ void moment_boxes(const DJob& J, const float4* __rest
       float part[Q];
 #pragma unroll
       for (int q = 0; q < Q; ++q) part[q] = 0.0f;
-#pragma unroll
+#pragma unroll 2
       for (int i = 0; i < 8; ++i) {
         const float4 v = L.cand[k + i];
 #pragma unroll
@@ -1238,7 +1238,10 @@ __device__ __forceinline__ void cell_moments(const DJob& J, const float* __restr
       float part[Q];
 #pragma unroll
       for (int q = 0; q < Q; ++q) part[q] = 0.0f;
-#pragma unroll
+      // one cell per step: unrolled, the LDS reads of several cells would be
+      // held in registers, raising the kernel's register count (and so
+      // lowering its occupancy)
+#pragma unroll 1
       for (int i = 0; i < 8; ++i) {
         // a = (N00, N01, N02, N10), b = (N11, N12, N20, N21), c = (N22, r0, r1, -)
         const float4 a = L.cand[k + i], b = L.cell[0][k + i], c = L.cell[1][k + i];
