@@ -291,8 +291,8 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   if (const char* m = std::getenv("ENT_BOX_MOMENTS"))
     if (!std::strcmp(m, "0")) moments = false;
   // 2-D jobs get a table of cells one row high and about h/4 wide in dim 0
-  // (m0/4 dim-0 boxes), when the job has a cell-start table and at most as
-  // many cells as points.
+  // (a power of two near m0/4 dim-0 boxes), when the job has a cell-start
+  // table and at most a quarter as many cells as points.
   uint64_t mom_total = 0, mom2_total = 0;
   h_mlist.assign(1, 0u);
   h_mlist2.assign(1, 0u);
@@ -314,10 +314,11 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
         h_mlist.push_back(waves);
         waves += (D.nb[0] + kMomentBoxesPerWave - 1) / kMomentBoxesPerWave;
       } else if (D.d == 2) {
-        const uint32_t cf = (uint32_t)std::max(1, tuning.m0 / 4);
+        uint32_t cf = 1;  // the power of two nearest below m0 / 4
+        while (cf * 2 <= (uint32_t)std::max(1, tuning.m0 / 4)) cf *= 2;
         const uint32_t ncb = (D.nb[0] + cf - 1) / cf;
         const uint64_t cells = (uint64_t)ncb * D.nb[1];
-        if (cells > D.n) continue;
+        if (4 * cells > D.n) continue;  // cells hold 4 points on average at least
         D.flags |= kJobMoments;
         D.mbase = mom2_total;
         D.mcf = cf;
@@ -337,7 +338,8 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   }
   const bool has_mom = mom_total != 0;
   const bool has_mom2 = mom2_total != 0;
-  const uint32_t mom2_blocks = (uint32_t)std::min<uint64_t>(P / kBlock + J, 1u << 30);
+  // grid bound of k_cell_moments2 (kBlock cells per block; cells <= points / 4)
+  const uint32_t mom2_blocks = (uint32_t)std::min<uint64_t>(P / (4 * kBlock) + J, 1u << 30);
   // grid bound of k_box_moments from the plan's shape: the listed boxes are at
   // most the points
   const uint32_t mom_waves = (uint32_t)std::min<uint64_t>(P / kMomentBoxesPerWave + J, 1u << 30);
