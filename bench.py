@@ -106,6 +106,9 @@ def roofline(st, kern_ms: float, workload: str) -> dict:
     flops = sum(p * f for p, f in zip(evaluated_pairs(st), FLOPS_PER_PAIR)) + sum(
         t * f for t, f in zip(st["moment_terms_by_d"], MOMENT_TERM_FLOPS))
     achieved = flops / (kern_ms * 1e-3) / 1e12
+    # the rate at which one-by-one evaluation of every candidate pair would
+    # have had to run to finish in the same time (not a hardware rate)
+    pair_equiv = sum(p * f for p, f in zip(st["pairs_by_d"], FLOPS_PER_PAIR)) / (kern_ms * 1e-3) / 1e12
     return {
         "bound": "valu",
         "kernel": "k_density (persistent)",
@@ -115,6 +118,7 @@ def roofline(st, kern_ms: float, workload: str) -> dict:
         "frac": achieved / PEAK_FP32_TFLOPS,
         "traffic": measured_traffic(workload),
         "kernel_ms": kern_ms,
+        "pair_equivalent_tflops": pair_equiv,
         "flops_per_pair": "4d per pair evaluated one by one (d=1: 4, d=2: 8, d=3: 12); "
                           "6 per (query, box) term of 1-D box moments, 25 per (query, cell) term of 2-D "
                           "cell moments",
