@@ -798,7 +798,8 @@ __global__ __launch_bounds__(kBlock) void k_box_moments(const DJob* __restrict__
   }
 }
 
-// Moment table of the 2-D jobs flagged kJobMoments, one thread per cell.
+// Moment table of the 2-D jobs flagged kJobMoments, one thread per cell
+// (points staged through LDS per block, see below).
 // Cell (row, cb) of job J covers dim-0 boxes [cb mcf, cb mcf + mcf) of one
 // row; with origin (r0, r1) = (fma(cb mcf, bs0, mn0), fma(row, bs1, mn1)) and
 // offsets (e_j, g_j) = (x_j - r0, y_j - r1) of its points, it stores the 9
@@ -814,6 +815,13 @@ __global__ __launch_bounds__(kBlock) void k_cell_moments2(const DJob* __restrict
                                                           const float* __restrict__ scoord,
                                                           const uint32_t* __restrict__ start,
                                                           float* __restrict__ mom) {
+  // The block's kBlock cells are consecutive in (row, cell) order, so their
+  // points are one contiguous stretch of the sorted arrays: it is staged
+  // through LDS kStage points at a time with coalesced loads, and each
+  // thread sums its own cell's points from LDS.
+  constexpr uint32_t kStage = 2048;
+  __shared__ float2 stage[kStage];
+  __shared__ uint32_t span[2];
   const uint32_t M = mlist[0];
   const uint32_t* bfirst = mlist + 1;
   const uint32_t* mjob = mlist + 2 + M;
@@ -826,54 +834,61 @@ __global__ __launch_bounds__(kBlock) void k_cell_moments2(const DJob* __restrict
       else hi = mid;
     }
     const DJob& J = jobs[mjob[lo]];
-    const uint32_t cell = (blk - bfirst[lo]) * kBlock + threadIdx.x;
+    const uint32_t cell0 = (blk - bfirst[lo]) * kBlock;
+    const uint32_t cell = cell0 + threadIdx.x;
     const uint32_t ncb = J.mncb, nb0 = J.nb[0];
-    if (cell >= J.nb[1] * ncb) continue;
-    const uint32_t row = cell / ncb;
-    const uint32_t cb = cell - row * ncb;
-    const uint32_t f0 = cb * J.mcf;
-    const uint32_t f1 = min(f0 + J.mcf, nb0);
-    const uint32_t* st = start + J.cbase + (uint64_t)row * nb0;
-    const uint32_t a = st[f0], e = st[f1];
-    const double r0 = (double)fmaf((float)f0, J.bs[0], J.mn[0]);
+    const uint32_t n_cells = J.nb[1] * ncb;
+    const bool valid = cell < n_cells;
+    uint32_t row = 0, cb = 0, a = 0, e = 0;
+    if (valid) {
+      row = cell / ncb;
+      cb = cell - row * ncb;
+      const uint32_t f0 = cb * J.mcf;
+      const uint32_t f1 = min(f0 + J.mcf, nb0);
+      const uint32_t* st = start + J.cbase + (uint64_t)row * nb0;
+      a = st[f0];
+      e = st[f1];
+    }
+    const uint32_t last = min(n_cells - cell0, (uint32_t)kBlock) - 1;
+    __syncthreads();  // the previous block's readers of span and stage are done
+    if (threadIdx.x == 0) span[0] = a;
+    if (threadIdx.x == last) span[1] = e;
+    __syncthreads();
+    const uint32_t A = span[0], B = span[1];
+    const double r0 = (double)fmaf((float)(cb * J.mcf), J.bs[0], J.mn[0]);
     const double r1 = (double)fmaf((float)row, J.bs[1], J.mn[1]);
     const float* x = scoord + 3 * J.pbase;
     const float* y = x + J.n;
     double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    auto add = [&](float xv, float yv) {
-      const double ex = (double)xv - r0, gy = (double)yv - r1;
-      const double g2 = gy * gy;
-      const double pw[3] = {1.0, ex, ex * ex};
+    for (uint32_t base = A; base < B; base += kStage) {
+      const uint32_t cnt = min(B - base, kStage);
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < cnt; i += kBlock) stage[i] = make_float2(x[base + i], y[base + i]);
+      __syncthreads();
+      const uint32_t p0 = max(a, base), p1 = min(e, base + cnt);
+      for (uint32_t p = p0; p < p1; ++p) {
+        const float2 v = stage[p - base];
+        const double ex = (double)v.x - r0, gy = (double)v.y - r1;
+        const double g2 = gy * gy;
+        const double pw[3] = {1.0, ex, ex * ex};
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        m[3 * i] += pw[i];
-        m[3 * i + 1] = fma(pw[i], gy, m[3 * i + 1]);
-        m[3 * i + 2] = fma(pw[i], g2, m[3 * i + 2]);
+        for (int i = 0; i < 3; ++i) {
+          m[3 * i] += pw[i];
+          m[3 * i + 1] = fma(pw[i], gy, m[3 * i + 1]);
+          m[3 * i + 2] = fma(pw[i], g2, m[3 * i + 2]);
+        }
       }
-    };
-    // eight points per step: their loads are issued together, so a step
-    // waits for one memory latency rather than eight
-    constexpr uint32_t kStep = 8;
-    uint32_t p = a;
-    for (; p + kStep <= e; p += kStep) {
-      float xv[kStep], yv[kStep];
-#pragma unroll
-      for (uint32_t i = 0; i < kStep; ++i) {
-        xv[i] = x[p + i];
-        yv[i] = y[p + i];
-      }
-#pragma unroll
-      for (uint32_t i = 0; i < kStep; ++i) add(xv[i], yv[i]);
     }
-    for (; p < e; ++p) add(x[p], y[p]);
-    const double k0 = (double)J.hinv[0] * (double)J.hinv[0];
-    const double k1 = (double)J.hinv[1] * (double)J.hinv[1];
-    const double s0[3] = {1.0, 2.0 * k0, -k0}, s1[3] = {1.0, 2.0 * k1, -k1};
-    float* out = mom + (J.mbase + cell) * 9;
+    if (valid) {
+      const double k0 = (double)J.hinv[0] * (double)J.hinv[0];
+      const double k1 = (double)J.hinv[1] * (double)J.hinv[1];
+      const double s0[3] = {1.0, 2.0 * k0, -k0}, s1[3] = {1.0, 2.0 * k1, -k1};
+      float* out = mom + (J.mbase + cell) * 9;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) out[3 * i + k] = (float)(m[3 * i + k] * s0[i] * s1[k]);
+        for (int k = 0; k < 3; ++k) out[3 * i + k] = (float)(m[3 * i + k] * s0[i] * s1[k]);
+    }
   }
 }
 
