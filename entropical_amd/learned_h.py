@@ -329,7 +329,7 @@ def evaluate_mi(ctx: Context, model: BandwidthModel, n_cols: int = 64, n_rows: i
     """The learned bandwidth on the MI workload's columns (64 x 200,000
     mixed columns, datasets.mixed_columns) against hestimate: per column the
     rule of thumb, the AMISE bandwidth and the network's choice (with its
-    AMISE fallback); each judged by the entropy it gives relative to the
+    AMISE fallback); each scored by the entropy it gives relative to the
     AMISE bandwidth's entropy, by its leave-one-out log-likelihood per point
     against the best of the 32-multiplier grid, and by the MI matrix it
     gives.  All densities on the GPU."""

@@ -265,7 +265,7 @@ int ent_ctx_last_driver_ms(ent_ctx* ctx, double* ms);
 
 /* ---- learned bandwidth model (no reference counterpart; the reference
  * chooses h with Hestimate::Thumb / AmiseMin, hestimate.cpp:19-104, which
- * stay available as the fallback and the judge).  A multilayer perceptron
+ * stay available as the fallback and the yardstick).  A multilayer perceptron
  * n_in -> hidden^n_hidden_layers -> 1 with tanh units, BF16 weights and FP32
  * accumulation, trained with minibatch Adam by this library's kernels. */
 typedef struct ent_mlp ent_mlp;
