@@ -336,8 +336,8 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     h_mlist2.push_back(blocks2);
     h_mlist2.insert(h_mlist2.end(), mj2.begin(), mj2.end());
   }
-  const bool has_mom = mom_total != 0;
-  const bool has_mom2 = mom2_total != 0;
+  bool has_mom = mom_total != 0;
+  bool has_mom2 = mom2_total != 0;
   // grid bound of k_cell_moments2 (kBlock cells per block; cells <= points / 4)
   const uint32_t mom2_blocks = (uint32_t)std::min<uint64_t>(P / (4 * kBlock) + J, 1u << 30);
   // grid bound of k_box_moments from the plan's shape: the listed boxes are at
@@ -530,8 +530,20 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   ENT_TRY(ensure(d_tiles, sizeof(DTile) * P));
   ENT_TRY(ensure(d_counter, sizeof(uint32_t) * kCounterWords));
   ENT_TRY(ensure(d_cellstart, sizeof(uint32_t) * std::max<uint64_t>(table_total, 1)));
-  if (has_mom) ENT_TRY(ensure(d_moments, sizeof(float4) * mom_total));
-  if (has_mom2) ENT_TRY(ensure(d_moments2, sizeof(float) * 9 * mom2_total));
+  if (has_mom || has_mom2) {
+    const bool ok = (!has_mom || ensure(d_moments, sizeof(float4) * mom_total) == kOk) &&
+                    (!has_mom2 || ensure(d_moments2, sizeof(float) * 9 * mom2_total) == kOk);
+    if (!ok) {
+      // no device memory left for the moment tables: every pair is evaluated
+      // one by one (the failed allocation is not a sticky error)
+      (void)hipGetLastError();
+      ENT_HIP(hipStreamSynchronize(stream_));
+      for (DJob& D : h_jobs) D.flags &= ~kJobMoments;
+      std::memcpy(host_stage_, h_jobs.data(), sizeof(DJob) * J);
+      ENT_HIP(hipMemcpyAsync(d_jobs.p, host_stage_, sizeof(DJob) * J, hipMemcpyHostToDevice, stream_));
+      has_mom = has_mom2 = false;
+    }
+  }
   if (shared) {
     for (int b = 0; b < 2; ++b) {
       ENT_TRY(ensure(d_bkeys[b], key_bytes * PB));
