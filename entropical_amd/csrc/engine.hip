@@ -597,15 +597,14 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
       if (with_tiles && npass == 0) launch_gather_sorted(stream_, dj, dch, NC, X, idx[cur], scoord);
     }
     if (!with_tiles) return;
-    if (table_total) launch_cell_starts(stream_, wide, dj, dch, NC, keys[cur], (uint32_t*)d_cellstart.p);
+    const uint2* tch = (const uint2*)d_tchunks.p;
+    launch_cell_starts(stream_, wide, dj, tch, NC, keys[cur], tq, tcnt, (uint32_t*)d_cellstart.p);
     if (has_mom)
       launch_box_moments(stream_, dj, (const uint32_t*)d_mlist.p, mom_waves, scoord, (const uint32_t*)d_cellstart.p,
                          (float4*)d_moments.p);
     if (has_mom2)
       launch_cell_moments2(stream_, dj, (const uint32_t*)d_mlist2.p, mom2_blocks, scoord,
                            (const uint32_t*)d_cellstart.p, (float*)d_moments2.p);
-    const uint2* tch = (const uint2*)d_tchunks.p;
-    launch_tile_count(stream_, wide, dj, tch, NC, keys[cur], tq, tcnt);
     launch_exclusive_scan(stream_, tcnt, tcnt, NC, scratch);
     launch_tile_write(stream_, wide, dj, tch, NC, keys[cur], tq, tcnt, (DTile*)d_tiles.p);
     launch_tile_ends(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);

@@ -680,62 +680,6 @@ __device__ __forceinline__ void load_key_pairs(const DJob& J, uint32_t y, int I0
     prev[i] = kk[pc > 0 ? pc - 1 : 0];
   }
 }
-template <typename KeyT>
-__global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__ jobs,
-                                                        const uint2* __restrict__ chunks,
-                                                        uint32_t n_chunks, const KeyT* __restrict__ keys,
-                                                        uint32_t* __restrict__ start) {
-  const uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-  if (c >= n_chunks) return;
-  const uint2 ch = chunks[c];
-  const DJob& J = jobs[ch.x];
-  if (J.cbase == kNoTable) return;
-  const int l = lane_id();
-  uint32_t* st = start + J.cbase;
-  const KeyT* kk = keys + J.pbase;
-  // Point p owns the cells (key[p-1], key[p]]; cells after the last key hold
-  // n.  Short runs are written by their own lane; runs longer than
-  // kLaneRun (empty stretches of a sparse grid) by the whole wave, one run
-  // at a time, so one long run does not serialise on a single lane.
-  constexpr uint64_t kLaneRun = 16;
-  constexpr int kHalf = kChunkItems / 2;
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    KeyT curk[kHalf], prevk[kHalf];
-    load_key_pairs<kHalf>(J, ch.y, h * kHalf, keys, l, curk, prevk);
-#pragma unroll
-    for (int ii = 0; ii < kHalf; ++ii) {
-      const uint32_t p0 = ch.y + (uint32_t)(h * kHalf + ii) * kWave;
-      if (p0 >= J.n) continue;
-      const uint32_t p = p0 + (uint32_t)l;
-      uint64_t first = 0, cnt = 0;
-      if (p < J.n) {
-        const uint64_t cur = curk[ii];
-        first = p == 0 ? 0 : (uint64_t)prevk[ii] + 1;
-        cnt = cur + 1 - first;
-      }
-      const bool wide_run = cnt > kLaneRun;
-      if (!wide_run)
-        for (uint64_t e = 0; e < cnt; ++e) st[first + e] = p;
-      uint64_t bm = __ballot(wide_run);
-      while (bm) {
-        const int src = __ffsll((unsigned long long)bm) - 1;
-        bm &= bm - 1;
-        const uint64_t f = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(first >> 32), src) << 32) |
-                           __builtin_amdgcn_readlane((uint32_t)first, src);
-        const uint64_t n = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(cnt >> 32), src) << 32) |
-                           __builtin_amdgcn_readlane((uint32_t)cnt, src);
-        const uint32_t v = p0 + (uint32_t)src;
-        for (uint64_t e = (uint64_t)l; e < n; e += kWave) st[f + e] = v;
-      }
-      if (J.n - 1 - p0 < (uint32_t)kWave) {  // this step holds the job's last point
-        const uint64_t tail = (uint64_t)kk[J.n - 1] + 1;
-        for (uint64_t cell = tail + (uint64_t)l; cell <= J.cells; cell += kWave) st[cell] = J.n;
-      }
-    }
-  }
-}
-
 // ------------------------------------------------------------ box moments
 // Moment table of the 1-D jobs flagged kJobMoments.  Box b of job J, with
 // origin r_b = fma(b, bs, mn) and the offsets e_j = x_j - r_b of its s0
@@ -928,23 +872,72 @@ __device__ __forceinline__ uint32_t chunk_tile_starts(const DJob& J, uint32_t y,
 }
 }  // namespace
 
+// Cell-start tables (jobs with a table) and per-chunk tile counts (every
+// job) in one pass over the sorted keys.  chunks are in tile order; the
+// tile count of chunk c goes to tile_cnt[c].
 template <typename KeyT>
-__global__ __launch_bounds__(kBlock) void k_tile_count(const DJob* __restrict__ jobs,
-                                                       const uint2* __restrict__ chunks,
-                                                       uint32_t n_chunks,
-                                                       const KeyT* __restrict__ keys,
-                                                       uint32_t tq, uint32_t* __restrict__ cnt) {
+__global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__ jobs,
+                                                        const uint2* __restrict__ chunks,
+                                                        uint32_t n_chunks, const KeyT* __restrict__ keys,
+                                                        uint32_t tq, uint32_t* __restrict__ tile_cnt,
+                                                        uint32_t* __restrict__ start) {
   const uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (c >= n_chunks) return;
   const uint2 ch = chunks[c];
   const DJob& J = jobs[ch.x];
+  const bool tab = J.cbase != kNoTable;
   const int l = lane_id();
-  const uint32_t bits = chunk_tile_starts(J, ch.y, keys, tq, l);
-  uint32_t total = 0;
+  uint32_t* st = start + (tab ? J.cbase : 0);
+  uint32_t tiles = 0;  // tile starts among the lane's points
+  const KeyT* kk = keys + J.pbase;
+  // Point p owns the cells (key[p-1], key[p]]; cells after the last key hold
+  // n.  Short runs are written by their own lane; runs longer than
+  // kLaneRun (empty stretches of a sparse grid) by the whole wave, one run
+  // at a time, so one long run does not serialise on a single lane.
+  constexpr uint64_t kLaneRun = 16;
+  constexpr int kHalf = kChunkItems / 2;
 #pragma unroll
-  for (int i = 0; i < kChunkItems; ++i) total += (uint32_t)__popcll(__ballot((bits >> i) & 1u));
-  if (l == 0) cnt[c] = total;
+  for (int h = 0; h < 2; ++h) {
+    KeyT curk[kHalf], prevk[kHalf];
+    load_key_pairs<kHalf>(J, ch.y, h * kHalf, keys, l, curk, prevk);
+#pragma unroll
+    for (int ii = 0; ii < kHalf; ++ii) {
+      const uint32_t p0 = ch.y + (uint32_t)(h * kHalf + ii) * kWave;
+      if (p0 >= J.n) continue;
+      const uint32_t p = p0 + (uint32_t)l;
+      uint64_t first = 0, cnt = 0;
+      if (p < J.n) {
+        if (tile_start(J, prevk[ii], curk[ii], p, tq)) ++tiles;
+        if (tab) {
+          const uint64_t cur = curk[ii];
+          first = p == 0 ? 0 : (uint64_t)prevk[ii] + 1;
+          cnt = cur + 1 - first;
+        }
+      }
+      const bool wide_run = cnt > kLaneRun;
+      if (!wide_run)
+        for (uint64_t e = 0; e < cnt; ++e) st[first + e] = p;
+      uint64_t bm = __ballot(wide_run);
+      while (bm) {
+        const int src = __ffsll((unsigned long long)bm) - 1;
+        bm &= bm - 1;
+        const uint64_t f = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(first >> 32), src) << 32) |
+                           __builtin_amdgcn_readlane((uint32_t)first, src);
+        const uint64_t n = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(cnt >> 32), src) << 32) |
+                           __builtin_amdgcn_readlane((uint32_t)cnt, src);
+        const uint32_t v = p0 + (uint32_t)src;
+        for (uint64_t e = (uint64_t)l; e < n; e += kWave) st[f + e] = v;
+      }
+      if (tab && J.n - 1 - p0 < (uint32_t)kWave) {  // this step holds the job's last point
+        const uint64_t tail = (uint64_t)kk[J.n - 1] + 1;
+        for (uint64_t cell = tail + (uint64_t)l; cell <= J.cells; cell += kWave) st[cell] = J.n;
+      }
+    }
+  }
+  const uint32_t total = __builtin_amdgcn_readlane(wave_incl_scan(tiles), kWave - 1);
+  if (l == 0) tile_cnt[c] = total;
 }
+
 
 template <typename KeyT>
 __global__ __launch_bounds__(kBlock) void k_tile_write(const DJob* __restrict__ jobs,
@@ -1915,16 +1908,6 @@ void launch_gather_sorted(hipStream_t s, const DJob* jobs, const uint2* chunks, 
                      idx, scoord);
 }
 
-void launch_tile_count(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const void* keys, uint32_t tq, uint32_t* cnt) {
-  if (wide)
-    hipLaunchKernelGGL(k_tile_count<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint64_t*)keys, tq, cnt);
-  else
-    hipLaunchKernelGGL(k_tile_count<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint32_t*)keys, tq, cnt);
-}
-
 void launch_tile_write(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
                        const void* keys, uint32_t tq, const uint32_t* offs, DTile* tiles) {
   if (wide)
@@ -2022,13 +2005,13 @@ void launch_cell_moments2(hipStream_t s, const DJob* jobs, const uint32_t* mlist
 }
 
 void launch_cell_starts(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                        const void* keys, uint32_t* start) {
+                        const void* keys, uint32_t tq, uint32_t* tile_cnt, uint32_t* start) {
   if (wide)
     hipLaunchKernelGGL(k_cell_starts<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint64_t*)keys, start);
+                       n_chunks, (const uint64_t*)keys, tq, tile_cnt, start);
   else
     hipLaunchKernelGGL(k_cell_starts<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint32_t*)keys, start);
+                       n_chunks, (const uint32_t*)keys, tq, tile_cnt, start);
 }
 
 }  // namespace ent

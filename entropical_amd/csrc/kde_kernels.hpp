@@ -34,9 +34,8 @@ void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uin
                            uint32_t* scratch);
 void launch_gather_sorted(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
                           const float* X, const uint32_t* idx, float* scoord);
-// Tile boundaries: every tq points, at row changes and between dim-0 groups.
-void launch_tile_count(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const void* keys, uint32_t tq, uint32_t* cnt);
+// Tile boundaries: every tq points, at row changes and between dim-0 groups
+// (counted per chunk by launch_cell_starts).
 void launch_tile_write(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
                        const void* keys, uint32_t tq, const uint32_t* offs, DTile* tiles);
 void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
@@ -72,7 +71,9 @@ constexpr uint32_t kMomentBoxesPerWave = 16;
 // max_blocks bounds the list's blocks.
 void launch_cell_moments2(hipStream_t s, const DJob* jobs, const uint32_t* mlist, uint32_t max_blocks,
                           const float* scoord, const uint32_t* start, float* mom);
+// Cell-start tables of the jobs that have one, and the tile count of every
+// chunk (chunks in tile order) into tile_cnt, in one pass over the keys.
 void launch_cell_starts(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                        const void* keys, uint32_t* start);
+                        const void* keys, uint32_t tq, uint32_t* tile_cnt, uint32_t* start);
 
 }  // namespace ent
