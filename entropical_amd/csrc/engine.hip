@@ -76,10 +76,12 @@ int Engine::ensure(DevBuf& b, size_t bytes) {
   if (b.p) ENT_HIP(hipFree(b.p));
   b.p = nullptr;
   b.cap = 0;
+  // captured graphs hold the old address: none is replayed from here on,
+  // whether or not the new allocation succeeds
+  ++generation_;
   size_t want = std::max<size_t>(bytes + bytes / 8, 4096);
   ENT_HIP(hipMalloc(&b.p, want));
   b.cap = want;
-  ++generation_;
   return kOk;
 }
 
