@@ -533,11 +533,15 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   ENT_TRY(ensure(d_counter, sizeof(uint32_t) * kCounterWords));
   ENT_TRY(ensure(d_cellstart, sizeof(uint32_t) * std::max<uint64_t>(table_total, 1)));
   if (has_mom || has_mom2) {
-    const bool ok = (!has_mom || ensure(d_moments, sizeof(float4) * mom_total) == kOk) &&
-                    (!has_mom2 || ensure(d_moments2, sizeof(float) * 9 * mom2_total) == kOk);
+    // ENT_MOMENT_TABLE_MAX (bytes) caps the moment tables' memory
+    uint64_t cap = ~0ull;
+    if (const char* m = std::getenv("ENT_MOMENT_TABLE_MAX")) cap = std::strtoull(m, nullptr, 10);
+    const uint64_t bytes1 = sizeof(float4) * mom_total, bytes2 = sizeof(float) * 9 * mom2_total;
+    const bool ok = bytes1 + bytes2 <= cap && (!has_mom || ensure(d_moments, bytes1) == kOk) &&
+                    (!has_mom2 || ensure(d_moments2, bytes2) == kOk);
     if (!ok) {
-      // no device memory left for the moment tables: every pair is evaluated
-      // one by one (the failed allocation is not a sticky error)
+      // moment tables over the cap or beyond the device memory left: every
+      // pair is evaluated one by one (a failed allocation is not a sticky error)
       (void)hipGetLastError();
       ENT_HIP(hipStreamSynchronize(stream_));
       for (DJob& D : h_jobs) D.flags &= ~kJobMoments;

@@ -112,8 +112,10 @@ int ent_ctx_last_pairs(ent_ctx* ctx, uint64_t* out4);
  * contributes a polynomial in the query's offset from its origin, whose
  * coefficients are the box's point count and moments up to second order
  * per dim: it equals the sum of the pairs' kernel values.
- * ENT_BOX_MOMENTS=0 in the environment turns this off.  No reference
- * counterpart. */
+ * ENT_BOX_MOMENTS=0 in the environment turns this off, and
+ * ENT_MOMENT_TABLE_MAX=<bytes> caps the tables' device memory (a run whose
+ * tables would exceed it, or the memory left, evaluates every pair one by
+ * one).  No reference counterpart. */
 int ent_ctx_last_moments(ent_ctx* ctx, uint64_t* out4);
 
 /* Device memory helpers on the context's device and stream. */

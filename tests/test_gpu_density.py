@@ -276,6 +276,30 @@ def test_cell_moments_2d(ctx, orc, monkeypatch, h):
     assert 0 < st1["moment_terms_by_d"][1] < st1["moment_pairs_by_d"][1]
 
 
+def test_moment_tables_over_cap_fall_back(ctx, orc, monkeypatch):
+    """Moment tables that do not fit (here: a 1-byte cap) leave every pair to
+    the pair loop: same densities within tolerance, no moment pairs, and the
+    next run without the cap uses moments again."""
+    from entropical_amd import combined_densities
+
+    r = np.random.default_rng(47)
+    n = 60000
+    a = r.standard_normal(n)
+    x = np.stack([a, 0.5 * a + r.standard_normal(n)]).astype(np.float32)
+    ref = orc.density_fp64(x, n, [0, 1], [0.2, 0.2])
+    monkeypatch.setenv("ENT_MOMENT_TABLE_MAX", "1")
+    capped = combined_densities(x, n, [0, 1], [0.2, 0.2], ctx=ctx)
+    st0 = ctx.last_stats()
+    monkeypatch.delenv("ENT_MOMENT_TABLE_MAX")
+    again = combined_densities(x, n, [0, 1], [0.2, 0.2], ctx=ctx)
+    st1 = ctx.last_stats()
+    assert rel_err(capped, ref) <= RTOL
+    assert rel_err(again, ref) <= RTOL
+    assert st0["moment_pairs"] == 0
+    assert st1["moment_pairs"] > 0
+    assert st0["pairs"] == st1["pairs"]
+
+
 # ---------------------------------------------------------- epa convolution
 def test_epa_convolution_matches_fp64(ctx, orc):
     from entropical_amd import epa_convolution
