@@ -5,6 +5,7 @@
 // The chunk table (job id, job-local start) is built on the host.
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 #include "kde_engine.hpp"
 #include "kde_kernels.hpp"
 
@@ -1061,6 +1062,16 @@ __device__ __forceinline__ uint32_t lower_bound_key(const KeyT* a, uint32_t n, u
   return lo;
 }
 
+// Tiles whose queries span more than this many bandwidths along dim 0 (sparse
+// stretches; tiles stay inside one dim-0 group, up to 64 bandwidths wide)
+// evaluate u = (x_q - x_c) / h from the raw coordinates.  Narrower tiles use
+// coordinates scaled about the centre of their bounding box, s = (x - o) / h,
+// and u = s_q - s_c: the FP32 rounding of s is about 1.2e-7 |s|, so with
+// |s| at most 17 (16 bandwidths from the centre plus the support) u stays
+// within about 4e-6 of its exact value.  (With the tile's first query as the
+// origin and 64-bandwidth tiles, a sparse query's density was off by 1.6e-5.)
+constexpr float kLocalSpan = 32.0f;
+
 struct WaveLds {
   float4 cand[kWave];        // staged candidates (local coordinates)
   uint2 rng[kWave];          // packed tiles: candidate ranges (first flattened index, first position)
@@ -1304,6 +1315,7 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
   }
 
   float o[3] = {0.f, 0.f, 0.f};
+  bool exact = false;  // raw-coordinate pair terms (a wide tile)
   uint32_t n_ranges = 1;
   uint32_t my_lo = 0, my_len = 0;
   uint32_t m_lo = 0, m_n = 0;  // 1-D: boxes summed from the moment table
@@ -1311,11 +1323,10 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
   if constexpr (RAW) {
     if (l == 0) { my_lo = 0; my_len = n; }
   } else {
-    // origin = first query of the tile; bounding box of the tile's queries
+    // bounding box of the tile's queries; its centre is the origin
     float bmin[3], bmax[3];
 #pragma unroll
     for (int e = 0; e < D; ++e) {
-      o[e] = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(xq[e][0])));
       float mn = xq[e][0], mx = xq[e][0];
 #pragma unroll
       for (int q = 1; q < Q; ++q) {
@@ -1324,7 +1335,9 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
       }
       bmin[e] = wave_min(mn);
       bmax[e] = wave_max(mx);
+      o[e] = 0.5f * (bmin[e] + bmax[e]);
     }
+    exact = (bmax[0] - bmin[0]) * J.hinv[0] > kLocalSpan;
     uint32_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 #pragma unroll
     for (int e = 0; e < D; ++e) box_range(bmin[e], bmax[e], J.h[e], J.mn[e], J.bs[e], J.nb[e], lo[e], hi[e]);
@@ -1451,7 +1464,7 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
 #pragma unroll
   for (int q = 0; q < Q; ++q)
 #pragma unroll
-    for (int e = 0; e < D; ++e) sq[e][q] = RAW ? xq[e][q] : (xq[e][q] - o[e]) * hinv[e];
+    for (int e = 0; e < D; ++e) sq[e][q] = (RAW || exact) ? xq[e][q] : (xq[e][q] - o[e]) * hinv[e];
 
   float S[Q], C[Q];
 #pragma unroll
@@ -1504,19 +1517,21 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
 
   // one staged chunk: convert the loaded candidates, stage them, reload the
   // buffer `ahead` chunks further on, run the pair loop
-  auto chunk = [&](uint32_t base, float* buf, uint32_t ahead) {
+  // exact_tag: std::true_type stages raw coordinates (u from raw differences)
+  auto chunk = [&](auto exact_tag, uint32_t base, float* buf, uint32_t ahead) {
+    constexpr bool kRawPairs = RAW || decltype(exact_tag)::value;
     float s_c[3];
 #pragma unroll
-    for (int e = 0; e < D; ++e) s_c[e] = RAW ? buf[e] : (buf[e] - o[e]) * hinv[e];
+    for (int e = 0; e < D; ++e) s_c[e] = kRawPairs ? buf[e] : (buf[e] - o[e]) * hinv[e];
     wave_sync();
     stage_store<D>(L, l, s_c);
     wave_sync();
     if (base + ahead < total) load_cand(base + ahead, buf);
     if constexpr (D == 1) {
-      if (tri) accumulate_chunk<1, Q, RAW, true>(L.cand, sq, hinv, S, C);
-      else accumulate_chunk<1, Q, RAW, false>(L.cand, sq, hinv, S, C);
+      if (tri) accumulate_chunk<1, Q, kRawPairs, true>(L.cand, sq, hinv, S, C);
+      else accumulate_chunk<1, Q, kRawPairs, false>(L.cand, sq, hinv, S, C);
     } else {
-      accumulate_chunk<D, Q, RAW>(L.cand, sq, hinv, S, C);
+      accumulate_chunk<D, Q, kRawPairs>(L.cand, sq, hinv, S, C);
     }
   };
   float nxt[3];
@@ -1536,7 +1551,10 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
       mstat[1] = c_nr * c_nc;
     }
   }
-  for (uint32_t base = 0; base < total; base += kWave) chunk(base, nxt, kWave);
+  if (!RAW && exact)
+    for (uint32_t base = 0; base < total; base += kWave) chunk(std::true_type{}, base, nxt, kWave);
+  else
+    for (uint32_t base = 0; base < total; base += kWave) chunk(std::false_type{}, base, nxt, kWave);
   wave_sync();
 
 #pragma unroll
@@ -1579,15 +1597,20 @@ __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __r
   const uint32_t slot = idx[J.pbase + p];
 
   // sub-tile bounding box (idle lanes repeat the sub-tile's first query);
-  // its lower corner is the sub-tile's origin
+  // its centre is the sub-tile's origin.  A pack with a sub-tile wider than
+  // kLocalSpan bandwidths along dim 0 uses raw-coordinate pair terms.
   float o[3] = {0.f, 0.f, 0.f};
   uint32_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  float span0 = 0.f;
 #pragma unroll
   for (int e = 0; e < D; ++e) {
-    o[e] = sub_min<W>(xq[e]);
+    const float mn = sub_min<W>(xq[e]);
     const float mx = sub_max<W>(xq[e]);
-    box_range(o[e], mx, J.h[e], J.mn[e], J.bs[e], J.nb[e], lo[e], hi[e]);
+    box_range(mn, mx, J.h[e], J.mn[e], J.bs[e], J.nb[e], lo[e], hi[e]);
+    o[e] = 0.5f * (mn + mx);
+    if (e == 0) span0 = (mx - mn) * J.hinv[0];
   }
+  const bool exact = wave_max(span0) > kLocalSpan;
   const uint32_t n1 = hi[1] - lo[1] + 1;
   const uint32_t n2 = (D == 3) ? (hi[2] - lo[2] + 1) : 1u;
   uint32_t nr = n1 * n2;
@@ -1637,7 +1660,7 @@ __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __r
 
   float sq[3][1];
 #pragma unroll
-  for (int e = 0; e < D; ++e) sq[e][0] = (xq[e] - o[e]) * hinv[e];
+  for (int e = 0; e < D; ++e) sq[e][0] = exact ? xq[e] : (xq[e] - o[e]) * hinv[e];
   float S[1] = {0.0f}, C[1] = {0.0f};
 
   // candidate f of the sub-tile: range k is the last one starting at or
@@ -1660,16 +1683,21 @@ __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __r
   const float4* stage = L.cand + s * (W / kPer4);
   float nxt[3];
   load_cand(0u, nxt);
-  for (uint32_t b = 0; b < steps; ++b) {
-    float s_c[3];
+  auto run = [&](auto exact_tag) {
+    constexpr bool kRawPairs = decltype(exact_tag)::value;
+    for (uint32_t b = 0; b < steps; ++b) {
+      float s_c[3];
 #pragma unroll
-    for (int e = 0; e < D; ++e) s_c[e] = (nxt[e] - o[e]) * hinv[e];
-    wave_sync();
-    stage_store<D>(L, l, s_c);
-    wave_sync();
-    if (b + 1 < steps) load_cand(b + 1, nxt);
-    accumulate_chunk<D, 1, false, false, W>(stage, sq, hinv, S, C);
-  }
+      for (int e = 0; e < D; ++e) s_c[e] = kRawPairs ? nxt[e] : (nxt[e] - o[e]) * hinv[e];
+      wave_sync();
+      stage_store<D>(L, l, s_c);
+      wave_sync();
+      if (b + 1 < steps) load_cand(b + 1, nxt);
+      accumulate_chunk<D, 1, kRawPairs, false, W>(stage, sq, hinv, S, C);
+    }
+  };
+  if (exact) run(std::true_type{});
+  else run(std::false_type{});
   wave_sync();
 
   if (qv) dens[J.out_off + slot] = ((double)S[0] - (double)C[0]) * J.norm;

@@ -300,6 +300,31 @@ def test_moment_tables_over_cap_fall_back(ctx, orc, monkeypatch):
     assert st0["pairs"] == st1["pairs"]
 
 
+@pytest.mark.parametrize("tuning", [(2, 32, 1, 64), (1, 32, 2, 64)])
+def test_wide_sparse_tiles(ctx, orc, tuning):
+    """Tiles in sparse stretches span up to a 64-bandwidth dim-0 group.  The
+    local coordinates of such tiles must keep the pair terms exact enough:
+    a narrow dim-0 bandwidth on normal data (whose tails give wide tiles;
+    1.6e-5 with the tile's first query as origin) and heavy-tailed data
+    (tiles wider than 32 bandwidths, raw-coordinate pair terms)."""
+    from entropical_amd import combined_densities
+
+    r = np.random.default_rng(109)
+    ctx.set_tuning(*tuning)
+    try:
+        n = 20000
+        x = r.standard_normal((2, n)).astype(np.float32)
+        for h in ([0.0432, 0.15972], [0.03, 0.1]):
+            got = combined_densities(x, n, [0, 1], h, ctx=ctx)
+            assert rel_err(got, orc.density_fp64_window(x, n, [0, 1], h)) < RTOL, h
+        y = (r.standard_t(1.5, size=(2, n)) * 3.0).astype(np.float32)
+        for cols, h in (([0], [0.05]), ([0, 1], [0.05, 0.5])):
+            got = combined_densities(y, n, cols, h, ctx=ctx)
+            assert rel_err(got, orc.density_fp64_window(y, n, cols, h)) < RTOL, (cols, h)
+    finally:
+        ctx.set_tuning()
+
+
 # ---------------------------------------------------------- epa convolution
 def test_epa_convolution_matches_fp64(ctx, orc):
     from entropical_amd import epa_convolution
