@@ -11,8 +11,12 @@
 
 // Register budget of the density kernel: at least this many waves per SIMD
 // (HIP's second launch-bounds argument), i.e. at most 512/N VGPRs per lane.
+// 6 keeps the two-queries-per-lane variant at 6 waves without spills (its
+// free allocation took 90 VGPRs, 5 waves); one query per lane is bound at 7
+// waves by its scalar registers either way, and four per lane spills (the
+// autotuner times every setting, so a slower one is not picked).
 #ifndef ENT_DENSITY_MIN_WAVES
-#define ENT_DENSITY_MIN_WAVES 1
+#define ENT_DENSITY_MIN_WAVES 6
 #endif
 
 
