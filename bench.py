@@ -13,15 +13,16 @@ Other workloads (--workload): te (32 coupled AR(1) series x 100,000 frames,
 h=0.2, lags 1,2,5,10), uneven (500 trajectories of 2,000-500,000 frames,
 d 1-3), entropy1d (N=100,000 three-component mixture, h=0.1).
 
-Printed JSON line: value = candidate pair evaluations per second
-(Gpair-evals/s) over the whole step: the (query, candidate) pairs of every
-tile's cell-list window, evaluated one by one or, for the boxes of a 1-D job
-and the cells of a 2-D job that lie inside the support of every query of a
-tile, summed exactly from their moments (both counts are reported); the
-dense-equivalent rate, the plain
-per-pair GPU path (the reference's per-call driver order, extrapolated from
-timed calls) and the host OpenMP baseline (oracle restatement of
-densities_omp.cpp on timed calls, extrapolated) are reported beside it.
+Printed JSON line: value = pair evaluations per second (Gpair-evals/s) over
+the whole step, counting the (query, candidate) pairs evaluated one by one.
+Beside it: the candidate rate (every pair of every tile's cell-list window,
+including the pairs of 1-D boxes and 2-D cells inside the support of every
+query of a tile, which are summed exactly from their moments), the
+dense-equivalent rate (sum of N^2), the plain per-pair GPU path (the
+reference's per-call driver order, extrapolated from timed calls) and the
+host OpenMP baseline (the oracle's restatement of densities_omp.cpp on a
+stratified sample of the reference's calls, extrapolated).  The host leg
+also checks sampled results against the FP64 oracle (max_rel_err_vs_fp64).
 Roofline: FP32 VALU; achieved = algorithmic FLOPs (4d per pair evaluated one
 by one, 6 per (query, box) and 25 per (query, cell) moment term) /
 duration of the persistent density kernel, measured with HIP events on the
@@ -51,12 +52,48 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def host_cores() -> int:
+# The GPU box grants one GPU's share of its host: 16 cores (its
+# OMP_NUM_THREADS), whatever the affinity mask shows.
+BOX_CPU_SHARE = 16
+
+
+def host_cores():
+    """(threads used, cores in the affinity mask, reason)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        mask = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        mask = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or BOX_CPU_SHARE
+    n = max(1, min(mask, share))
+    why = (f"{n} threads: the host share granted with one GPU (OMP_NUM_THREADS={share}); "
+           f"the affinity mask holds {mask} cores")
+    return n, mask, why
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_oracle(threads: int):
+    """The oracle (test infrastructure) for the host leg: OpenMP threads bound
+    close (OMP_PROC_BIND is read when the runtime starts, at first load)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    import oracle
+
+    oracle.set_threads(threads)
+    return oracle
+
+
+def rel(a: float, b: float) -> float:
+    return abs(a - b) / abs(b)
 
 
 def time_call(fn, reps: int = 3) -> float:
@@ -161,16 +198,45 @@ def timed_steps(ctx, step, steps: int, warmup: int):
     return elapsed / steps, float(np.mean(dens_ms)), st, float(np.mean(util))
 
 
-def cpu_omp_time(packed: np.ndarray, n: int, h, threads: int) -> float:
-    import oracle
+def cpu_omp_time(orc, packed: np.ndarray, n: int, h) -> float:
+    return time_call(lambda: orc.density_omp_packed(packed, n, h), reps=1)
 
-    oracle.set_threads(threads)
-    return time_call(lambda: oracle.density_omp_packed(packed, n, h), reps=1)
+
+def cpu_baseline_info(threads, mask, why, kind_note):
+    return {"cores": threads, "kind": "port", "cores_note": why, "cpu_model": cpu_model(),
+            "affinity_cores": mask, "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
+            "method": kind_note}
+
+
+def rates(st, sec: float) -> dict:
+    """Pair counts of the last run and their rates over one step."""
+    ev = sum(evaluated_pairs(st))
+    cand = sum(st["pairs_by_d"])
+    return {"pairs_evaluated": ev, "pairs_candidate": cand, "pairs_by_moments": st["moment_pairs"],
+            "moment_terms": st["moment_terms"], "pairs_by_dim": st["pairs_by_d"],
+            "pairs_evaluated_by_dim": evaluated_pairs(st), "candidate_gpair_per_s": cand / sec / 1e9}
 
 
 # ------------------------------------------------------------------ MI
+def mi_call_sequence(n_cols: int) -> list:
+    """The columns of every combined_densities call of Mi::main, in its order
+    (mi.cpp:37-52): p_i, then for every j > i p_j and p_ij."""
+    seq = []
+    for i in range(n_cols):
+        seq.append((i,))
+        for j in range(i + 1, n_cols):
+            seq.append((j,))
+            seq.append((i, j))
+    return seq
+
+
+def stratified(seq: list, k: int) -> list:
+    """k calls spread evenly over a call sequence."""
+    return [seq[(2 * m + 1) * len(seq) // (2 * k)] for m in range(k)] if k > 0 else []
+
+
 def run_mi(args) -> dict:
-    from entropical_amd import Context, DeviceArray, combined_densities, datasets, mi_autotune, mi_matrix
+    from entropical_amd import Context, DeviceArray, batch_entropy, combined_densities, datasets, mi_autotune, mi_matrix
 
     n_cols, n_rows, h = 64, 200_000, 0.15
     log("generating 64 x 200000 mixed columns")
@@ -183,11 +249,12 @@ def run_mi(args) -> dict:
     sec, kern_ms, st, util = timed_steps(ctx, lambda: mi_matrix(ctx, dx, hs), args.steps, args.warmup)
     mi = mi_matrix(ctx, dx, hs)
     assert np.all(np.isfinite(mi)) and np.allclose(mi, mi.T)
-    pairs = sum(st["pairs_by_d"])
-    log(f"MI matrix {1e3 * sec:.1f} ms/step, density kernel {kern_ms:.1f} ms, {pairs:.3e} pairs")
+    r = rates(st, sec)
+    log(f"MI matrix {1e3 * sec:.1f} ms/step, density kernel {kern_ms:.1f} ms, {r['pairs_evaluated']:.3e} pairs "
+        f"evaluated ({r['pairs_candidate']:.3e} candidates)")
     res = {
-        "metric": "Gpair-evals/s (batched 64-column MI matrix, 200k frames)",
-        "value": pairs / sec / 1e9,
+        "metric": "Gpair-evals/s (pairs evaluated one by one; batched 64-column MI matrix, 200k frames)",
+        "value": r["pairs_evaluated"] / sec / 1e9,
         "unit": "Gpair-evals/s",
         "ms_per_step": 1e3 * sec,
         "config": {
@@ -196,12 +263,14 @@ def run_mi(args) -> dict:
             "n_cols": n_cols, "n_rows": n_rows, "bandwidth": h, "method": "cell list",
             "tuning_q_m0_mr_group": list(tun),
         },
-        "pairs_candidate": pairs, "pairs_by_moments": st["moment_pairs"], "moment_terms": st["moment_terms"],
-        "pairs_by_dim": st["pairs_by_d"],
+        **r,
         "lane_utilisation": util,
         "dense_equiv_gpair_per_s": 2080 * n_rows * n_rows / sec / 1e9,
         "roofline": roofline(st, kern_ms, "mi"),
     }
+    # results the host leg checks against FP64 (outside the timed region)
+    check_pairs = ((0, 1), (17, 42), (5, 63), (30, 31))
+    H0 = float(batch_entropy(ctx, dx.ptr, [{"n_dim": 1, "n": n_rows, "n_rows": n_rows, "off": [0], "h": [h]}])[0])
     if not args.no_plain:
         log("timing plain per-pair path (one call each, extrapolated to mi.cpp's 4096 calls)")
         plain = {}
@@ -216,18 +285,35 @@ def run_mi(args) -> dict:
     dx.free()
     ctx.close()
     if not args.no_cpu_baseline:
-        th = args.cpu_threads or host_cores()
-        t1 = cpu_omp_time(X[0:1], n_rows, [h], th)
-        log(f"cpu 1-D density {t1:.2f} s")
-        t2 = cpu_omp_time(X[0:2], n_rows, [h, h], th)
-        log(f"cpu 2-D density {t2:.2f} s")
-        total = 2080 * t1 + 2016 * t2
+        th, mask, why = host_cores()
+        orc = host_oracle(th)
+        seq = mi_call_sequence(n_cols)
+        one = [c for c in seq if len(c) == 1]
+        two = [c for c in seq if len(c) == 2]
+        k1, k2 = args.cpu_calls
+        t1 = [cpu_omp_time(orc, X[list(c)], n_rows, [h]) for c in stratified(one, k1)]
+        log(f"cpu 1-D densities {np.mean(t1):.2f} s each ({k1} calls)")
+        t2 = [cpu_omp_time(orc, X[list(c)], n_rows, [h, h]) for c in stratified(two, k2)]
+        log(f"cpu 2-D densities {np.mean(t2):.2f} s each ({k2} calls)")
+        total = len(one) * float(np.mean(t1)) + len(two) * float(np.mean(t2))
         res["cpu_baseline"] = {
-            "value": total, "unit": "s per MI matrix (extrapolated)", "cores": th, "kind": "port",
-            "sample": f"one 1-D and one 2-D density at N={n_rows} ({t1:.2f} s, {t2:.2f} s) with the oracle's "
-                      "restatement of densities_omp.cpp, scaled by mi.cpp's 2080 + 2016 calls",
+            "value": total, "unit": "s per MI matrix (extrapolated)",
+            **cpu_baseline_info(th, mask, why, "oracle restatement of densities_omp.cpp (sorted-window scan, "
+                                "OpenMP rows), -O3 -fopenmp -ftree-vectorize"),
+            "sample": f"{k1} of mi.cpp's {len(one)} 1-D and {k2} of its {len(two)} 2-D combined_densities calls "
+                      f"at N={n_rows}, spread evenly over its call order (mean {np.mean(t1):.2f} s and "
+                      f"{np.mean(t2):.2f} s per call), extrapolated to all 4096 calls",
         }
         res["speedup_vs_cpu"] = total / sec
+        log("FP64 checks of sampled results")
+        cols = sorted({c for ij in check_pairs for c in ij} | {0})
+        p = {c: orc.density_fp64_window(X, n_rows, [c], [h]) for c in cols}
+        errs = {}
+        for i, j in check_pairs:
+            ref = orc.mi_fp64(orc.density_fp64_window(X, n_rows, [i, j], [h, h]), p[i], p[j])
+            errs[f"mi[{i},{j}]"] = rel(mi[i, j], ref)
+        errs["entropy[col 0]"] = rel(H0, orc.entropy(p[0]))
+        res["max_rel_err_vs_fp64"] = {"max": max(errs.values()), "bound": 1e-4, "results": errs}
     return res
 
 
@@ -237,25 +323,26 @@ def run_te(args) -> dict:
 
     n_cols, n_rows, h, taus = 32, 100_000, 0.2, [1, 2, 5, 10]
     log("generating 32 coupled AR(1) series x 100000")
-    X, _, _ = datasets.coupled_ar1(n_cols, n_rows)
+    X, drv, _ = datasets.coupled_ar1(n_cols, n_rows)
     hs = [h] * n_cols
     ctx = Context(0)
     dx = DeviceArray(ctx, X)
     tun = te_autotune(ctx, dx, hs, taus) if not args.no_tune else ctx.get_tuning()
     log(f"tuning (q, m0, mr, group_h) = {tun}")
     sec, kern_ms, st, util = timed_steps(ctx, lambda: te_matrix(ctx, dx, hs, taus), args.steps, args.warmup)
-    pairs = sum(st["pairs_by_d"])
-    log(f"TE (4 lags) {1e3 * sec:.1f} ms/step, density kernel {kern_ms:.1f} ms, {pairs:.3e} pairs")
+    te = te_matrix(ctx, dx, hs, taus)
+    r = rates(st, sec)
+    log(f"TE (4 lags) {1e3 * sec:.1f} ms/step, density kernel {kern_ms:.1f} ms, {r['pairs_evaluated']:.3e} pairs")
     n_pairs = n_cols * (n_cols - 1) // 2
     jobs_3d, jobs_2d, jobs_1d = 2 * n_pairs * len(taus), n_pairs + n_cols * len(taus), n_cols
     dense = (jobs_3d + jobs_2d + jobs_1d) * float(n_rows) ** 2
     res = {
-        "metric": "Gpair-evals/s (transfer entropy, 32 series x 100k frames, 4 lags)",
-        "value": pairs / sec / 1e9, "unit": "Gpair-evals/s", "ms_per_step": 1e3 * sec,
+        "metric": "Gpair-evals/s (pairs evaluated one by one; transfer entropy, 32 series x 100k frames, 4 lags)",
+        "value": r["pairs_evaluated"] / sec / 1e9, "unit": "Gpair-evals/s", "ms_per_step": 1e3 * sec,
         "config": {"workload": "te_32series_100000frames_h0.2_tau1-2-5-10",
                    "jobs": f"{jobs_3d} 3-D + {jobs_2d} 2-D + {jobs_1d} 1-D densities, one persistent launch",
                    "tuning_q_m0_mr_group": list(tun)},
-        "pairs_candidate": pairs, "pairs_by_moments": st["moment_pairs"], "moment_terms": st["moment_terms"], "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
+        **r, "lane_utilisation": util,
         "dense_equiv_gpair_per_s": dense / sec / 1e9,
         "roofline": roofline(st, kern_ms, "te"),
     }
@@ -275,17 +362,37 @@ def run_te(args) -> dict:
     dx.free()
     ctx.close()
     if not args.no_cpu_baseline:
-        th = args.cpu_threads or host_cores()
-        tc = {1: cpu_omp_time(X[0:1], n_rows, [h], th), 2: cpu_omp_time(X[0:2], n_rows, [h, h], th)}
+        th, mask, why = host_cores()
+        orc = host_oracle(th)
+        tc = {1: cpu_omp_time(orc, X[[0]], n_rows, [h]), 2: cpu_omp_time(orc, X[[0, 1]], n_rows, [h, h])}
         lag = np.stack([X[0, :n_rows - 5], X[0, 5:], X[1, :n_rows - 5]])
-        tc[3] = cpu_omp_time(lag, n_rows - 5, [h, h, h], th)
+        tc[3] = cpu_omp_time(orc, lag, n_rows - 5, [h, h, h])
         total = sum(calls[d] * tc[d] for d in (1, 2, 3))
         res["cpu_baseline"] = {
-            "value": total, "unit": "s per 4-lag TE matrix (extrapolated)", "cores": th, "kind": "port",
+            "value": total, "unit": "s per 4-lag TE matrix (extrapolated)",
+            **cpu_baseline_info(th, mask, why, "oracle restatement of densities_omp.cpp (packed columns, lag "
+                                "applied), -O3 -fopenmp -ftree-vectorize"),
             "sample": "one 1-D, 2-D and 3-D density at N=100000 ({:.2f}, {:.2f}, {:.2f} s), scaled by the "
                       "reference's per-lag call counts".format(tc[1], tc[2], tc[3]),
         }
         res["speedup_vs_cpu"] = total / sec
+        log("FP64 checks of sampled results")
+        i = 5
+        j = int(drv[i])
+        a, b = min(i, j), max(i, j)
+        tau = taus[0]
+        m = n_rows - tau
+        f = lambda v: float(np.float32(v))
+        pa = orc.density_fp64_window(X, n_rows, [a], [h])
+        pb = orc.density_fp64_window(X, n_rows, [b], [h])
+        pab = orc.density_fp64_window(X, n_rows, [a, b], [h, h])
+        errs = {}
+        for (u, v, pu, cols, lags) in ((a, b, pa, [a, a, b], [0, tau, 0]), (b, a, pb, [a, b, b], [0, 0, tau])):
+            p3 = orc.density_fp64_window(X, n_rows, cols, [h, h, h], lags)
+            ptau = orc.density_fp64_window(X, n_rows, [u, u], [h, h], [0, tau])
+            ref = orc.te_fp64(p3, pu[:m], pab[:m], ptau, f(h) ** 3)
+            errs[f"te[tau={tau}][{u},{v}]"] = rel(te[0, u, v], ref)
+        res["max_rel_err_vs_fp64"] = {"max": max(errs.values()), "bound": 1e-4, "results": errs}
     return res
 
 
@@ -301,20 +408,44 @@ def run_uneven(args) -> dict:
     tun = batch_autotune(ctx, ptr, jobs) if not args.no_tune else ctx.get_tuning()
     log(f"tuning (q, m0, mr, group_h) = {tun}")
     sec, kern_ms, st, util = timed_steps(ctx, lambda: batch_entropy(ctx, ptr, jobs), args.steps, args.warmup)
-    pairs = sum(st["pairs_by_d"])
+    H = batch_entropy(ctx, ptr, jobs)
+    r = rates(st, sec)
     dense = float(sum(j["n"] ** 2 for j in jobs))
-    log(f"uneven batch {1e3 * sec:.1f} ms/step, density kernel {kern_ms:.1f} ms, {pairs:.3e} pairs")
+    log(f"uneven batch {1e3 * sec:.1f} ms/step, density kernel {kern_ms:.1f} ms, {r['pairs_evaluated']:.3e} pairs")
     ctx.free(ptr)
     ctx.close()
-    return {
-        "metric": "Gpair-evals/s (500 uneven trajectories, densities + entropies)",
-        "value": pairs / sec / 1e9, "unit": "Gpair-evals/s", "ms_per_step": 1e3 * sec,
+    res = {
+        "metric": "Gpair-evals/s (pairs evaluated one by one; 500 uneven trajectories, densities + entropies)",
+        "value": r["pairs_evaluated"] / sec / 1e9, "unit": "Gpair-evals/s", "ms_per_step": 1e3 * sec,
         "config": {"workload": "uneven_500traj_2k-500k_d1-3", "points": int(sum(j["n"] for j in jobs)),
                    "tuning_q_m0_mr_group": list(tun)},
-        "pairs_candidate": pairs, "pairs_by_moments": st["moment_pairs"], "moment_terms": st["moment_terms"], "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
+        **r, "lane_utilisation": util,
         "dense_equiv_gpair_per_s": dense / sec / 1e9,
         "roofline": roofline(st, kern_ms, "uneven"),
     }
+    if not args.no_cpu_baseline:
+        th, mask, why = host_cores()
+        orc = host_oracle(th)
+        log("FP64 checks of sampled results")
+        errs = {}
+        for d in (1, 2):
+            k = max((s for s in range(len(jobs)) if jobs[s]["n_dim"] == d), key=lambda s: jobs[s]["n"])
+            s = jobs[k]
+            pts = np.stack([buf[s["off"][e]: s["off"][e] + s["n"]] for e in range(d)])
+            t0 = time.perf_counter()
+            orc.density_omp_packed(pts, s["n"], s["h"][:d])
+            errs[f"cpu_{d}d_s"] = time.perf_counter() - t0
+            p = orc.density_fp64_window(pts, s["n"], list(range(d)), s["h"][:d])
+            errs[f"entropy[traj {k}, d={d}, n={s['n']}]"] = rel(H[k], orc.entropy(p))
+        t1, t2 = errs.pop("cpu_1d_s"), errs.pop("cpu_2d_s")
+        res["max_rel_err_vs_fp64"] = {"max": max(errs.values()), "bound": 1e-4, "results": errs}
+        res["cpu_baseline"] = {
+            "value": t1 + t2, "unit": "s for the longest 1-D and 2-D trajectories",
+            **cpu_baseline_info(th, mask, why, "oracle restatement of densities_omp.cpp"),
+            "sample": "the batch's longest 1-D and 2-D trajectory ({:.2f} s, {:.2f} s); the whole batch is "
+                      "not extrapolated (its cost is not linear in the sample)".format(t1, t2),
+        }
+    return res
 
 
 # ------------------------------------------------------------- entropy1d
@@ -330,27 +461,38 @@ def run_entropy1d(args) -> dict:
     tun = batch_autotune(ctx, ptr, spec) if not args.no_tune else ctx.get_tuning()
     sec, kern_ms, st, util = timed_steps(ctx, lambda: batch_entropy(ctx, ptr, spec), args.steps, args.warmup)
     H = float(batch_entropy(ctx, ptr, spec)[0])
-    pairs = sum(st["pairs_by_d"])
+    dens = combined_densities(x, n, [0], [h], ctx=ctx)
+    r = rates(st, sec)
     res = {
-        "metric": "Gpair-evals/s (1-D KDE entropy, N=100k)", "value": pairs / sec / 1e9, "unit": "Gpair-evals/s",
+        "metric": "Gpair-evals/s (pairs evaluated one by one; 1-D KDE entropy, N=100k)",
+        "value": r["pairs_evaluated"] / sec / 1e9, "unit": "Gpair-evals/s",
         "ms_per_step": 1e3 * sec,
         "config": {"workload": "entropy1d_gmm3_100000_h0.1", "tuning_q_m0_mr_group": list(tun)},
-        "entropy": H, "pairs_candidate": pairs, "pairs_by_moments": st["moment_pairs"], "moment_terms": st["moment_terms"], "pairs_by_dim": st["pairs_by_d"], "lane_utilisation": util,
+        "entropy": H, **r, "lane_utilisation": util,
         "dense_equiv_gpair_per_s": float(n) ** 2 / sec / 1e9, "roofline": roofline(st, kern_ms, "entropy1d"),
     }
     if not args.no_plain:
         combined_densities(x, n, [0], [h], method="brute", ctx=ctx)
         tb = time_call(lambda: combined_densities(x, n, [0], [h], method="brute", ctx=ctx))
-        res["plain_path"] = {"brute": {"t_1d_ms": 1e3 * tb}}
+        tc = time_call(lambda: combined_densities(x, n, [0], [h], method="cell", ctx=ctx))
+        res["plain_path"] = {"brute": {"t_1d_ms": 1e3 * tb}, "cell": {"t_1d_ms": 1e3 * tc}}
         res["speedup_vs_plain_brute"] = tb / sec
+        res["speedup_vs_plain_cell"] = tc / sec
     ctx.free(ptr)
     ctx.close()
     if not args.no_cpu_baseline:
-        th = args.cpu_threads or host_cores()
-        t = cpu_omp_time(x, n, [h], th)
-        res["cpu_baseline"] = {"value": t, "unit": "s per density", "cores": th, "kind": "port",
+        th, mask, why = host_cores()
+        orc = host_oracle(th)
+        t = cpu_omp_time(orc, x, n, [h])
+        res["cpu_baseline"] = {"value": t, "unit": "s per density",
+                               **cpu_baseline_info(th, mask, why, "oracle restatement of densities_omp.cpp"),
                                "sample": "the full workload (one 1-D density, N=100000)"}
         res["speedup_vs_cpu"] = t / sec
+        ref = orc.density_fp64_window(x, n, [0], [h])
+        errs = {"density (max over 100000 points)": float(np.max(np.abs(dens - ref) / ref)),
+                "entropy": rel(H, orc.entropy(ref))}
+        res["max_rel_err_vs_fp64"] = {"max": errs["entropy"], "bound": 1e-4, "results": errs,
+                                      "density_bound": 1e-5}
     return res
 
 
@@ -366,7 +508,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-calls", type=lambda v: tuple(int(a) for a in v.split(",")), default=(4, 12),
+                    help="MI host baseline: 1-D,2-D calls timed (16,64 is the sample of the measurement plan)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a copy kernel of known traffic (counter calibration) and exit")
     args = ap.parse_args(argv)

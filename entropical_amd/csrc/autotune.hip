@@ -21,12 +21,18 @@ namespace ent {
 namespace {
 
 // Workload signature: per dimension, the number of jobs and their total
-// points rounded to two significant digits.
-std::string signature(const std::vector<JobSpec>& jobs) {
+// points rounded to two significant digits, and per (job dimension, axis)
+// the mean of log2(data range / bandwidth) in steps of 1/2 (the tile shapes
+// are in units of the bandwidth, so the same jobs at another bandwidth or
+// data scale are tuned again).  range_h[j * 3 + e]: range / h of job j, axis e.
+std::string signature(const std::vector<JobSpec>& jobs, const std::vector<double>& range_h) {
   uint64_t cnt[3] = {0, 0, 0}, pts[3] = {0, 0, 0};
-  for (const JobSpec& j : jobs) {
-    cnt[j.d - 1]++;
-    pts[j.d - 1] += j.n;
+  double lsum[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const JobSpec& s = jobs[j];
+    cnt[s.d - 1]++;
+    pts[s.d - 1] += s.n;
+    for (uint32_t e = 0; e < s.d; ++e) lsum[s.d - 1][e] += std::log2(std::max(range_h[3 * j + e], 1e-3));
   }
   std::ostringstream os;
   for (int d = 0; d < 3; ++d) {
@@ -34,7 +40,10 @@ std::string signature(const std::vector<JobSpec>& jobs) {
     double r = p > 0 ? std::round(p / std::pow(10.0, std::floor(std::log10(p)) - 1)) *
                            std::pow(10.0, std::floor(std::log10(p)) - 1)
                      : 0.0;
-    os << "d" << d + 1 << ":" << cnt[d] << ":" << (uint64_t)r << ";";
+    os << "d" << d + 1 << ":" << cnt[d] << ":" << (uint64_t)r;
+    for (int e = 0; e <= d && cnt[d] > 0; ++e)
+      os << ":r" << std::round(2.0 * lsum[d][e] / (double)cnt[d]) / 2.0;
+    os << ";";
   }
   return os.str();
 }
@@ -55,7 +64,37 @@ int Engine::autotune(const float* X, const std::vector<JobSpec>& jobs, double* d
   const size_t want = std::max<size_t>(8, jobs.size() / 8);
   const size_t stride = std::max<size_t>(1, jobs.size() / want);
   for (size_t i = 0; i < jobs.size(); i += stride) sample.push_back(jobs[i]);
-  const std::string sig = signature(sample);
+
+  // The caller's tuning is restored when the search fails; grid points that
+  // the workload does not support (too many boxes per dim at a fine box
+  // size, a workspace beyond the device memory) are skipped.
+  const Tuning saved = tuning;
+  auto fail = [&](int rc) {
+    tuning = saved;
+    return rc;
+  };
+  auto skippable = [](int rc) { return rc == kErrUnsupported || rc == kErrOom; };
+
+  // data range over bandwidth per job and axis, from a planning pass (the
+  // coarsest boxes of the grid when the caller's tuning is too fine)
+  std::vector<double> range_h(3 * sample.size(), 1.0);
+  {
+    int rc = densities(X, sample, dens, kMethodCell, kStopAfterPlan);
+    if (skippable(rc)) {
+      (void)hipGetLastError();
+      Tuning coarse = saved;
+      coarse.m0 = 8;
+      coarse.mr = 1;
+      tuning = coarse;
+      rc = densities(X, sample, dens, kMethodCell, kStopAfterPlan);
+    }
+    if (rc != kOk) return fail(rc);
+    for (size_t j = 0; j < sample.size(); ++j)
+      for (uint32_t e = 0; e < h_jobs[j].d; ++e)
+        range_h[3 * j + e] = (double)h_jobs[j].nb[e] * (double)h_jobs[j].bs[e] / (double)h_jobs[j].h[e];
+    tuning = saved;
+  }
+  const std::string sig = signature(sample, range_h);
 
   if (!tune_cache_loaded_) {
     tune_cache_loaded_ = true;
@@ -105,9 +144,17 @@ int Engine::autotune(const float* X, const std::vector<JobSpec>& jobs, double* d
           t.group_h = gh;
           double ms = 0.0;
           int rc = eval(t, 1, &ms, sample);
-          if (rc != kOk) return rc;
+          if (skippable(rc)) {
+            (void)hipGetLastError();
+            continue;
+          }
+          if (rc != kOk) return fail(rc);
           scored.emplace_back(ms, t);
         }
+  if (scored.empty()) {
+    set_error("autotune: no tile shape of the grid supports this workload");
+    return fail(kErrUnsupported);
+  }
   std::sort(scored.begin(), scored.end(),
             [](const std::pair<double, Tuning>& a, const std::pair<double, Tuning>& b) { return a.first < b.first; });
   Tuning cur = scored[0].second;
@@ -115,11 +162,19 @@ int Engine::autotune(const float* X, const std::vector<JobSpec>& jobs, double* d
   for (size_t k = 0; k < std::min<size_t>(4, scored.size()); ++k) {
     double ms = 0.0;
     int rc = eval(scored[k].second, 2, &ms, jobs);
-    if (rc != kOk) return rc;
+    if (skippable(rc)) {
+      (void)hipGetLastError();
+      continue;
+    }
+    if (rc != kOk) return fail(rc);
     if (ms < cur_ms) {
       cur_ms = ms;
       cur = scored[k].second;
     }
+  }
+  if (cur_ms >= 1e300) {
+    set_error("autotune: no tile shape of the grid supports the whole batch");
+    return fail(kErrUnsupported);
   }
   tuning = cur;
   tune_cache_[sig] = cur;

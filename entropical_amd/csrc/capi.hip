@@ -221,6 +221,17 @@ int ent_ctx_last_moments(ent_ctx* ctx, uint64_t* out4) {
   return ENT_OK;
 }
 
+int ent_ctx_last_tiles(ent_ctx* ctx, uint64_t* out2) {
+  CAPI_TRY(check_ctx(ctx));
+  if (!out2) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  out2[0] = ctx->eng->last.tiles;
+  out2[1] = ctx->eng->last.raw_tiles;
+  return ENT_OK;
+}
+
 int ent_dev_alloc(ent_ctx* ctx, size_t bytes, void** ptr) {
   CAPI_TRY(check_ctx(ctx));
   if (!ptr) {

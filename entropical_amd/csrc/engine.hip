@@ -624,7 +624,12 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   PlanKey key = plan_key(X, dens, J, NC, P, npass, false);
   key.NCB = shared ? NCB : 0;
   key.PB = shared ? PB : 0;
-  key.variant = (shared ? 1 : 0) | (staged ? 2 : 0) | (pack ? 4 : 0) | (npass0 << 3) | (has_mom ? 256 : 0) | (has_mom2 ? 512 : 0);
+  // the key width selects the kernel variants of every captured launch (and
+  // the buffers' element size), so it is part of the shape: a batch whose
+  // keys change width at unchanged pass counts must not replay the other
+  // width's graph
+  key.variant = (shared ? 1 : 0) | (staged ? 2 : 0) | (pack ? 4 : 0) | (npass0 << 3) | (has_mom ? 256 : 0) |
+                (has_mom2 ? 512 : 0) | (wide ? 1024 : 0);
   ENT_TRY(run_captured(pre_graphs_, key, [&] { issue_pre(true); }));
   return launch_density_common(false, X, dens, P);
 }
@@ -765,13 +770,13 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
   }
   ENT_HIP(hipEventRecord(ev_[2], stream_));
   // stats readback
-  ENT_TRY(ensure_host(80));
+  ENT_TRY(ensure_host(96));
   uint32_t* hc = (uint32_t*)host_stage_;
-  ENT_HIP(hipMemcpyAsync(hc, counter, 72, hipMemcpyDeviceToHost, stream_));
-  ENT_HIP(hipMemcpyAsync(hc + 18, n_tiles, 4, hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipMemcpyAsync(hc, counter, 80, hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipMemcpyAsync(hc + 20, n_tiles, 4, hipMemcpyDeviceToHost, stream_));
   ENT_HIP(hipStreamSynchronize(stream_));
-  uint64_t pd[8];
-  std::memcpy(pd, hc + 2, 64);
+  uint64_t pd[9];
+  std::memcpy(pd, hc + 2, 72);
   last.lane_pairs = pd[3];
   last.pairs = pd[0] + pd[1] + pd[2];
   for (int d = 0; d < 3; ++d) last.pairs_by_d[d] = pd[d];
@@ -779,7 +784,8 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
     last.moment_pairs[d] = pd[4 + d];
     last.moment_terms[d] = pd[6 + d];
   }
-  last.tiles = hc[18];
+  last.raw_tiles = pd[8];
+  last.tiles = hc[20];
   float ms = 0.f;
   ENT_HIP(hipEventElapsedTime(&ms, ev_[1], ev_[2]));
   last.density_ms = ms;

@@ -70,6 +70,7 @@ struct RunStats {
                                       // summed from box / cell moments
   uint64_t moment_terms[2] = {0, 0};  // (query, box) terms of 1-D jobs, (query, cell) of 2-D jobs
   uint64_t tiles = 0;
+  uint64_t raw_tiles = 0;  // tiles (and packed tiles) whose pair terms used raw coordinates
   uint32_t radix_passes = 0;  // base passes + job passes
   uint32_t sort_bases = 0;    // shared dim-0 sorts (0: one sort per job by full key)
   uint64_t table_cells = 0;  // cell-start table entries of the last plan

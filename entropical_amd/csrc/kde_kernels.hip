@@ -1286,14 +1286,15 @@ __device__ __forceinline__ void cell_moments(const DJob& J, const float* __restr
 }
 
 // Densities of one tile's queries; returns the tile's pair-by-pair candidate
-// count (wave-uniform).  mstat (1-D jobs with a moment table): candidates
-// summed from box moments and the number of those boxes.
+// count (wave-uniform).  mstat[0..1] (jobs with a moment table): candidates
+// summed from moments and the number of those boxes or cells; mstat[2]: 1
+// when the tile's pair terms were taken from raw coordinates (a wide tile).
 template <int D, int Q, bool RAW, typename KeyT>
 __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __restrict__ X,
                                  const float* __restrict__ scoord, const KeyT* __restrict__ keys,
                                  const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ idx,
                                  const float4* __restrict__ mom, const float* __restrict__ mom2,
-                                 double* __restrict__ dens, WaveLds& L, uint32_t (&mstat)[2]) {
+                                 double* __restrict__ dens, WaveLds& L, uint32_t (&mstat)[3]) {
   const int l = lane_id();
   const uint32_t n = J.n;
   const float* src[3];
@@ -1555,6 +1556,7 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
       mstat[1] = c_nr * c_nc;
     }
   }
+  if (!RAW && exact) mstat[2] = 1u;
   if (!RAW && exact)
     for (uint32_t base = 0; base < total; base += kWave) chunk(std::true_type{}, base, nxt, kWave);
   else
@@ -1577,7 +1579,7 @@ template <int D, int G, typename KeyT>
 __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __restrict__ scoord,
                                  const KeyT* __restrict__ keys, const uint32_t* __restrict__ cstart,
                                  const uint32_t* __restrict__ idx, double* __restrict__ dens, WaveLds& L,
-                                 uint64_t& lane_slots) {
+                                 uint64_t& lane_slots, uint64_t& raw_tiles) {
   constexpr int W = kWave / G;
   constexpr int kPer4 = D == 2 ? 2 : 1;
   const int l = lane_id();
@@ -1700,8 +1702,12 @@ __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __r
       accumulate_chunk<D, 1, kRawPairs, false, W>(stage, sq, hinv, S, C);
     }
   };
-  if (exact) run(std::true_type{});
-  else run(std::false_type{});
+  if (exact) {
+    ++raw_tiles;
+    run(std::true_type{});
+  } else {
+    run(std::false_type{});
+  }
   wave_sync();
 
   if (qv) dens[J.out_off + slot] = ((double)S[0] - (double)C[0]) * J.norm;
@@ -1718,14 +1724,15 @@ __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __r
 // filled tiles do not pay for idle slots.
 // pairs counts every candidate pair of the tile; macc[d-1] those summed from
 // moments in d-dim jobs (d = 1, 2), macc[d+1] the (query, box) / (query,
-// cell) moment terms evaluated for them.
+// cell) moment terms evaluated for them, macc[4] the tiles with raw-coordinate
+// pair terms.
 template <int D, int QMAX, bool RAW, typename KeyT>
 __device__ __forceinline__ void run_tile(const DJob& J, const DTile& T, const float* __restrict__ X,
                                          const float* __restrict__ scoord, const KeyT* __restrict__ keys,
                                          const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ idx,
                                          const float4* __restrict__ mom, const float* __restrict__ mom2,
                                          double* __restrict__ dens, WaveLds& L, uint64_t& pairs,
-                                         uint64_t& lane_slots, uint64_t (&macc)[4]) {
+                                         uint64_t& lane_slots, uint64_t (&macc)[5]) {
   const uint32_t nq = T.p1 - T.p0;
   if constexpr (QMAX >= 2 && !RAW) {  // plain-path tiles are full except the last
     if (nq <= (uint32_t)(kWave * (QMAX - 1))) {
@@ -1734,7 +1741,7 @@ __device__ __forceinline__ void run_tile(const DJob& J, const DTile& T, const fl
       return;
     }
   }
-  uint32_t mstat[2] = {0u, 0u};
+  uint32_t mstat[3] = {0u, 0u, 0u};
   const uint32_t total =
       process_tile<D, QMAX, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, mom2, dens, L, mstat);
   pairs += (uint64_t)nq * (total + mstat[0]);
@@ -1743,6 +1750,7 @@ __device__ __forceinline__ void run_tile(const DJob& J, const DTile& T, const fl
     macc[D - 1] += (uint64_t)nq * mstat[0];
     macc[D + 1] += (uint64_t)nq * mstat[1];
   }
+  macc[4] += mstat[2];
 }
 
 }  // namespace
@@ -1769,7 +1777,7 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
   const int l = lane_id();
   // candidate pairs per dimension, lane slots and the moment counts, flushed
   // once per wave
-  uint64_t acc[3] = {0, 0, 0}, slots = 0, macc[4] = {0, 0, 0, 0};
+  uint64_t acc[3] = {0, 0, 0}, slots = 0, macc[5] = {0, 0, 0, 0, 0};
   // Tiles are claimed from kClaimParts contiguous partitions of the tile
   // list, each with its own counter: a wave starts on the partition of its
   // workgroup's XCD (workgroups are dealt to the XCDs round robin) and moves
@@ -1796,11 +1804,11 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
         if (pk & 3u) {
           const bool four = (pk & 3u) == 3u;
           if (d == 2)
-            acc[1] += four ? process_pack<2, 4, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots)
-                           : process_pack<2, 2, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots);
+            acc[1] += four ? process_pack<2, 4, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots, macc[4])
+                           : process_pack<2, 2, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots, macc[4]);
           else
-            acc[2] += four ? process_pack<3, 4, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots)
-                           : process_pack<3, 2, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots);
+            acc[2] += four ? process_pack<3, 4, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots, macc[4])
+                           : process_pack<3, 2, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots, macc[4]);
           continue;
         }
       }
@@ -1816,7 +1824,7 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
     for (int e = 0; e < 3; ++e)
       if (acc[e]) atomicAdd(pairs + e, (unsigned long long)acc[e]);
     if (slots) atomicAdd(pairs + 3, (unsigned long long)slots);
-    for (int e = 0; e < 4; ++e)
+    for (int e = 0; e < 5; ++e)
       if (macc[e]) atomicAdd(pairs + 4 + e, (unsigned long long)macc[e]);
   }
 }

@@ -45,7 +45,7 @@ void launch_tile_pack(hipStream_t s, const DJob* jobs, DTile* tiles, const uint3
                       uint64_t max_tiles);
 // Claim counters of the persistent density kernel: kClaimParts counters
 // kClaimStride words apart (separate memory channels), one per partition of
-// the tile list; words 2..9 hold the pair statistics.  The buffer has
+// the tile list; words 2..19 hold the pair statistics (nine uint64).  The buffer has
 // kCounterWords words, all cleared before each launch.
 #ifndef ENT_CLAIM_PARTS
 #define ENT_CLAIM_PARTS 8

@@ -86,11 +86,14 @@ class Context:
         check(load().ent_ctx_last_sort(self.handle, srt))
         mom = (C.c_uint64 * 4)()
         check(load().ent_ctx_last_moments(self.handle, mom))
+        tl = (C.c_uint64 * 2)()
+        check(load().ent_ctx_last_tiles(self.handle, tl))
         return {"density_ms": dm.value, "total_ms": tm.value, "pairs": pairs.value, "tiles": tiles.value,
                 "pairs_by_d": [int(v) for v in by_d[:3]], "lane_pairs": int(by_d[3]),
                 "radix_passes": int(srt[0]), "sort_bases": int(srt[1]),
                 "moment_pairs": int(mom[0]) + int(mom[1]), "moment_pairs_by_d": [int(mom[0]), int(mom[1])],
-                "moment_terms": int(mom[2]) + int(mom[3]), "moment_terms_by_d": [int(mom[2]), int(mom[3])]}
+                "moment_terms": int(mom[2]) + int(mom[3]), "moment_terms_by_d": [int(mom[2]), int(mom[3])],
+                "raw_tiles": int(tl[1])}
 
     # device memory -------------------------------------------------------
     def alloc(self, nbytes: int) -> int:
@@ -400,10 +403,15 @@ def te_autotune(ctx: Context, x: DeviceArray, h: Sequence[float], taus: Sequence
     return tuple(int(v) for v in t)
 
 
-def batch_autotune(ctx: Context, x_dev_ptr: int, specs: Sequence[dict]) -> tuple:
-    """Tune the engine for an arbitrary batch of jobs (kept on ``ctx``)."""
+def batch_autotune(ctx: Context, x_dev_ptr: int, specs: Sequence[dict], return_ms: bool = False):
+    """Tune the engine for an arbitrary batch of jobs (kept on ``ctx``).
+
+    Returns the tuning (q, m0, mr, group_h); with ``return_ms`` also the
+    winner's time on the whole batch in ms (-1 when the tuning came from the
+    cache of an earlier search of the same workload signature)."""
     arr = make_jobs(specs)
     t = (C.c_int * 4)()
     ms = C.c_double()
     check(load().ent_autotune_dev(ctx.handle, C.c_void_p(x_dev_ptr), arr, len(specs), t, C.byref(ms)))
-    return tuple(int(v) for v in t)
+    tun = tuple(int(v) for v in t)
+    return (tun, ms.value) if return_ms else tun

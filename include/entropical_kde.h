@@ -118,6 +118,13 @@ int ent_ctx_last_pairs(ent_ctx* ctx, uint64_t* out4);
  * one).  No reference counterpart. */
 int ent_ctx_last_moments(ent_ctx* ctx, uint64_t* out4);
 
+/* Tiles of the last cell-list run: out[0] = tiles claimed by the density
+ * kernel (packed tiles count once), out[1] = those whose pair terms were
+ * taken from raw coordinates, u = (x_q - x_c) / h, because their queries
+ * span more than 32 bandwidths along dim 0 (sparse stretches); the others
+ * use coordinates scaled about the tile's centre.  No reference counterpart. */
+int ent_ctx_last_tiles(ent_ctx* ctx, uint64_t* out2);
+
 /* Device memory helpers on the context's device and stream. */
 int ent_dev_alloc(ent_ctx* ctx, size_t bytes, void** ptr);
 int ent_dev_free(ent_ctx* ctx, void* ptr);

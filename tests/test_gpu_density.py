@@ -321,7 +321,60 @@ def test_wide_sparse_tiles(ctx, orc, tuning):
         for cols, h in (([0], [0.05]), ([0, 1], [0.05, 0.5])):
             got = combined_densities(y, n, cols, h, ctx=ctx)
             assert rel_err(got, orc.density_fp64_window(y, n, cols, h)) < RTOL, (cols, h)
+            # the heavy tails give tiles wider than 32 bandwidths: the
+            # raw-coordinate pair terms ran
+            assert ctx.last_stats()["raw_tiles"] > 0, (cols, h)
+        # dim-0 groups of 16 bandwidths bound every tile's span below 32
+        # bandwidths: no tile takes the raw-coordinate path
+        ctx.set_tuning(tuning[0], tuning[1], tuning[2], 16)
+        for cols, h in (([0], [0.05]), ([0, 1], [0.05, 0.5])):
+            got = combined_densities(y, n, cols, h, ctx=ctx)
+            assert rel_err(got, orc.density_fp64_window(y, n, cols, h)) < RTOL, (cols, h)
+            assert ctx.last_stats()["raw_tiles"] == 0, (cols, h)
     finally:
+        ctx.set_tuning()
+
+
+def test_key_width_change_replays_right_graph(ctx, orc, monkeypatch):
+    """A shared-sort batch whose cell keys go from 64-bit (more than 2^32
+    cells) to 32-bit and back at unchanged radix pass counts, on one context
+    with the same coordinate and density buffers: each run's captured
+    launch sequence must be the one of its own key width.  Jobs this sparse
+    have no cell-start table, so the density kernel finds candidate ranges
+    by binary search over the keys, which a key of the other width breaks."""
+    from entropical_amd import DeviceArray, batch_densities_dev
+
+    r = np.random.default_rng(131)
+    n_base, n_cols = 1500, 4
+    base = r.uniform(0.0, 1.0, size=(n_cols, n_base))
+    # each point has a partner closer than every bandwidth used below
+    x = np.concatenate([base, base + np.array([2e-5, 5e-6, 5e-6, 5e-6])[:, None]], axis=1)
+    x = np.clip(x, 0.0, 1.0)
+    x[:, 0], x[:, 1] = 0.0, 1.0  # range exactly 1 in every column
+    x = x.astype(np.float32)
+    n = x.shape[1]
+    # m0 = 8, mr = 1: dim-0 boxes = 8 / h0, row boxes = 1 / h1.
+    # wide:   2^17 x 2^16 cells (34 bits), passes 3 (dim 0) + 2 (rows)
+    # narrow: 81920 x 32768 cells (32 bits), passes 3 + 2
+    wide_h = (2.0 ** -14, 2.0 ** -16)
+    narrow_h = (8.0 / 81920.0, 2.0 ** -15)
+    monkeypatch.setenv("ENT_SORT", "shared")
+    ctx.set_tuning(2, 8, 1, 4)
+    dx = DeviceArray(ctx, x)
+    out = ctx.alloc(8 * 3 * n)
+    try:
+        for h0, h1 in (wide_h, narrow_h, wide_h, narrow_h):
+            jobs = [{"n_dim": 2, "n": n, "n_rows": n, "off": [0, c * n], "h": [h0, h1], "out_off": (c - 1) * n}
+                    for c in (1, 2, 3)]
+            batch_densities_dev(ctx, dx.ptr, jobs, out)
+            got = np.empty(3 * n)
+            ctx.d2h(got, out)
+            for c in (1, 2, 3):
+                ref = orc.density_fp64(x, n, [0, c], [h0, h1])
+                assert rel_err(got[(c - 1) * n:c * n], ref) < RTOL, (h0, h1, c)
+    finally:
+        ctx.free(out)
+        dx.free()
         ctx.set_tuning()
 
 

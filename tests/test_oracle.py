@@ -294,6 +294,100 @@ def test_te_guards(orc):
     assert strange == 1
 
 
+def test_te_fp64_conditional_independence_is_zero(orc):
+    # p(x_t+tau, x_t, y_t) = p(x_t+tau, x_t) p(x_t, y_t) / p(x_t): the ratio
+    # p3 p_i / (p_ij p_itau) is 1 at every point, so TE vanishes exactly
+    r = rng(31)
+    p1 = r.uniform(0.2, 0.9, 200)
+    p2 = r.uniform(0.1, 0.5, 200)
+    pt = r.uniform(0.1, 0.5, 200)
+    p3 = p2 * pt / p1
+    assert np.all(p3 <= 1.0)
+    assert orc.te_fp64(p3, p1, p2, pt, pref=0.3) == pytest.approx(0.0, abs=1e-15)
+
+
+def test_te_fp64_hand_computed(orc):
+    # transs.cpp:75-100: sum over p3 > 0 of p3 * pref * log(p3 p_i inv(p_ij) inv(p_itau)),
+    # rejected when p3 > 1 (or not finite)
+    p3 = np.array([0.5, 0.3, 2.0, 0.0, 0.2])
+    p1 = np.array([0.6, 0.4, 1.0, 0.7, 0.1])     # p_i: numerator
+    p2 = np.array([0.25, 0.0, 1.0, 0.3, 0.4])    # p_ij: inv(0) = 0 -> ratio 0 -> term 0
+    pt = np.array([0.8, 0.5, 0.1, 0.2, 0.05])    # p_itau: denominator
+    h_i, h_j = 0.5, 2.0
+    pref = h_i * h_i * h_j
+    # row 0: ratio 0.5*0.6/(0.25*0.8) = 1.5; row 1: inv(0); row 2: p3 > 1 (would be
+    # 2*log(20)); row 3: p3 = 0 skipped; row 4: ratio 0.2*0.1/(0.4*0.05) = 1.0
+    expect = 0.5 * pref * math.log(1.5)
+    got = orc.te_fp64(p3, p1, p2, pt, pref=pref)
+    assert got == pytest.approx(expect, rel=1e-14)
+    assert got > 0
+    # the same sum in bits
+    assert orc.te_fp64(p3, p1, p2, pt, pref=pref, bits=True) == pytest.approx(
+        0.5 * pref * math.log2(1.5), rel=1e-14)
+    # a flipped ratio, or p_i and p_itau exchanged, changes the value
+    assert orc.te_fp64(p3, pt, p2, p1, pref=pref) != pytest.approx(expect, rel=1e-3)
+    assert orc.te_fp64(p3, p2, p1, pt, pref=pref) != pytest.approx(expect, rel=1e-3)
+
+
+def test_te_fp64_agrees_with_reference_arithmetic(orc):
+    # te_ref (pinned by hand above) rounds the ratio to float and uses logf;
+    # te_fp64 is the same sum in FP64.  Random inputs including the guard rows.
+    r = rng(32)
+    n = 5000
+    p3 = r.uniform(0.0, 0.9, n)
+    p1 = r.uniform(0.05, 1.0, n)
+    p2 = r.uniform(0.05, 1.0, n)
+    pt = r.uniform(0.05, 1.0, n)
+    p3[::97] = 0.0
+    p2[::89] = 0.0
+    pt[::83] = 0.0
+    p3[::101] = 1.5
+    a = orc.te_fp64(p3, p1, p2, pt, pref=0.7)
+    b, strange = orc.te_ref(p3, p1, p2, pt, pref=0.7)
+    assert strange == np.count_nonzero(p3 > 1.0)
+    assert a == pytest.approx(b, rel=1e-6)
+
+
+def test_autoent_hand_computed(orc):
+    # autoent.cpp:52-57: sum over p2 > 0 of p2 h h log(p1 inv(p2 h))
+    h = 0.5
+    p1 = np.array([0.3, 0.5, 0.2])
+    p2 = np.array([0.2, 0.0, 0.8])
+    # row 0: 0.2*0.25*log(0.3/0.1) ; row 1 skipped (p2 = 0); row 2: 0.8*0.25*log(0.2/0.4)
+    expect = 0.2 * 0.25 * math.log(3.0) + 0.8 * 0.25 * math.log(0.5)
+    got = orc.autoent_ref(p1, p2, h)
+    assert got == pytest.approx(expect, rel=1e-6)
+    assert orc.autoent_ref(p1, p2, h, bits=True) == pytest.approx(
+        0.2 * 0.25 * math.log2(3.0) + 0.8 * 0.25 * math.log2(0.5), rel=1e-6)
+    # arguments exchanged: a different sum
+    assert orc.autoent_ref(p2, p1, h) != pytest.approx(expect, rel=1e-3)
+
+
+def test_autoent_closed_form_zero(orc):
+    # p1 = p2 h at every point: each log argument is 1
+    r = rng(33)
+    h = 0.25
+    p2 = r.uniform(0.1, 2.0, 300)
+    assert orc.autoent_ref(p2 * h, p2, h) == pytest.approx(0.0, abs=1e-6)
+
+
+def test_kldiv_asymmetric_hand_computed(orc):
+    # kldiv.cpp:49-51: KL[i,j] = sum p_i log(p_i / p_j), KL[j,i] = sum p_j log(p_j / p_i)
+    p = np.array([0.25, 0.75])
+    q = np.array([0.5, 0.5])
+    kl_pq = 0.25 * math.log(0.5) + 0.75 * math.log(1.5)
+    kl_qp = 0.5 * math.log(2.0) + 0.5 * math.log(2.0 / 3.0)
+    a, b = orc.kldiv_ref(p, q)
+    assert a == pytest.approx(kl_pq, rel=1e-6)
+    assert b == pytest.approx(kl_qp, rel=1e-6)
+    assert abs(kl_pq - kl_qp) > 1e-2  # the two directions differ
+    a2, b2 = orc.kldiv_ref(q, p)
+    assert a2 == pytest.approx(kl_qp, rel=1e-6) and b2 == pytest.approx(kl_pq, rel=1e-6)
+    a, b = orc.kldiv_ref(p, q, bits=True)
+    assert a == pytest.approx(kl_pq / math.log(2.0), rel=1e-6)
+    assert b == pytest.approx(kl_qp / math.log(2.0), rel=1e-6)
+
+
 def test_entropy_uniform(orc):
     p = np.full(10, 2.0)
     assert orc.entropy(p) == pytest.approx(-math.log(2.0), rel=1e-15)
