@@ -23,7 +23,7 @@ LIB = PKG / "libentropical_kde.so"
 ARCH = os.environ.get("ENT_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["kde_kernels.hip", "engine.hip", "autotune.hip", "capi.hip", "reductions.hip", "drivers.hip",
-               "mlp.hip"]
+               "mlp.hip", "noalloc.hip"]
 HOST_SOURCES = ["densities_shim.cpp", "xtc.cpp"]
 
 HIPCC_FLAGS = [
@@ -56,7 +56,18 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{out}")
 
 
-def build_library(jobs: int = 8, verbose: bool = True) -> Path:
+def _stale(src: Path, obj: Path) -> bool:
+    """An object older than its source or any header of the library."""
+    if not obj.exists():
+        return True
+    t = obj.stat().st_mtime
+    deps = [src, *CSRC.glob("*.hpp"), *(ROOT / "include").rglob("*.h*")]
+    return any(d.stat().st_mtime > t for d in deps)
+
+
+def build_library(jobs: int = 8, verbose: bool = True, incremental: bool = False) -> Path:
+    """Compile every source (incremental: only those whose object is stale)
+    and link the library."""
     hipcc = _hipcc()
     BUILD.mkdir(exist_ok=True)
     cmds = []
@@ -64,15 +75,17 @@ def build_library(jobs: int = 8, verbose: bool = True) -> Path:
     for src in HIP_SOURCES:
         obj = BUILD / (src + ".o")
         objs.append(obj)
-        cmds.append([hipcc, *HIPCC_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
+        if not incremental or _stale(CSRC / src, obj):
+            cmds.append([hipcc, *HIPCC_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
     for src in HOST_SOURCES:
         if not (CSRC / src).exists():
             continue
         obj = BUILD / (src + ".o")
         objs.append(obj)
-        cmds.append([hipcc, "-x", "c++", "-O3", "-std=c++17", "-fPIC", "-Wall",
-                     "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}", f"-I{CSRC}",
-                     "-c", str(CSRC / src), "-o", str(obj)])
+        if not incremental or _stale(CSRC / src, obj):
+            cmds.append([hipcc, "-x", "c++", "-O3", "-std=c++17", "-fPIC", "-Wall",
+                         "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}", f"-I{CSRC}",
+                         "-c", str(CSRC / src), "-o", str(obj)])
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB.with_suffix(".so.tmp")
@@ -121,8 +134,9 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--oracle", action="store_true")
     ap.add_argument("-j", type=int, default=8)
+    ap.add_argument("--incremental", action="store_true", help="recompile only stale objects")
     a = ap.parse_args(argv)
-    build_library(a.j)
+    build_library(a.j, incremental=a.incremental)
     build_dropin_check()
     build_cli()
     if a.oracle:
