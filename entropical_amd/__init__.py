@@ -23,6 +23,7 @@ from .kde import (  # noqa: F401
     bin_column,
     cell_sort,
     combined_densities,
+    combined_densities_dev,
     default_context,
     device_count,
     epa_convolution,
@@ -33,6 +34,7 @@ __all__ = [
     "EntropicalError",
     "Context",
     "combined_densities",
+    "combined_densities_dev",
     "epa_convolution",
     "prob_dens_coord_prep",
     "bin_column",

@@ -82,6 +82,7 @@ SIGNATURES = {
     "ent_coord_prep": (C.c_int, [P, C.c_size_t, P, P, P, C.c_uint32, C.c_int, P, C.POINTER(C.c_size_t)]),
     "ent_combined_densities": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P, P, C.c_uint32, C.c_int, P]),
     "ent_combined_densities_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P, P, C.c_uint32, C.c_int, P]),
+    "ent_combined_densities_dev_noalloc": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P, P, C.c_uint32, C.c_int, P]),
     "ent_batch_densities_dev": (C.c_int, [P, P, C.POINTER(ent_job), C.c_size_t, C.c_int, P]),
     "ent_epa_convolution": (C.c_int, [P, P, C.c_size_t, C.c_float, F32P]),
     "ent_bin": (C.c_int, [P, P, C.c_size_t, C.c_float, P, U32P, F32P, F32P]),

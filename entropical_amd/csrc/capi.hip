@@ -38,10 +38,12 @@ int check_ctx(ent_ctx* ctx) {
     if (r_ != ENT_OK) return r_; \
   } while (0)
 
+}  // namespace
+
 // Argument checks of prob_dens_coord_prep (densities_common.hxx:15-33) plus
 // the conditions the reference leaves undefined (tau_max >= n_rows, column
 // index out of range, non-positive bandwidth).
-int validate_density_args(size_t n_rows, size_t n_cols, const uint32_t* i_cols, const float* h,
+int ent_validate_density_args(size_t n_rows, size_t n_cols, const uint32_t* i_cols, const float* h,
                           const uint32_t* tau, uint32_t n_dim, uint32_t* tau_max_out) {
   if (n_dim < 1 || n_dim > 3) {
     set_error("error: can only compute combined probabilities in 1, 2 or 3 dimensions.");
@@ -75,6 +77,11 @@ int validate_density_args(size_t n_rows, size_t n_cols, const uint32_t* i_cols, 
   return ENT_OK;
 }
 
+namespace {
+int validate_density_args(size_t n_rows, size_t n_cols, const uint32_t* i_cols, const float* h,
+                          const uint32_t* tau, uint32_t n_dim, uint32_t* tau_max_out) {
+  return ent_validate_density_args(n_rows, n_cols, i_cols, h, tau, n_dim, tau_max_out);
+}
 }  // namespace
 
 extern "C" {
@@ -122,6 +129,27 @@ int ent_ctx_create(int device, ent_ctx** out) {
     set_error("error: could not initialise the HIP device");
     return ENT_ERR_NO_DEVICE;
   }
+  // set-up of the no-allocation entry point: the device's scratch array and
+  // an engine that carves its workspace from it (host staging allocated now)
+  void* sp = nullptr;
+  size_t sb = 0;
+  int rc = ent_noalloc_scratch(device, &sp, &sb);
+  if (rc == ENT_OK) {
+    ctx->eng_na = new (std::nothrow) ent::Engine(device);
+    if (!ctx->eng_na || !ctx->eng_na->stream()) {
+      set_error("error: could not initialise the HIP device");
+      rc = ENT_ERR_NO_DEVICE;
+    } else {
+      rc = ctx->eng_na->use_arena(sp, sb, 1u << 20);
+    }
+  }
+  if (rc == ENT_OK) ctx->na_jobs.reserve(1);
+  if (rc != ENT_OK) {
+    delete ctx->eng_na;
+    delete ctx->eng;
+    delete ctx;
+    return rc;
+  }
   *out = ctx;
   return ENT_OK;
 }
@@ -137,6 +165,7 @@ int ent_ctx_destroy(ent_ctx* ctx) {
     if (ctx->aux.p) (void)hipFree(ctx->aux.p);
     if (ctx->red.p) (void)hipFree(ctx->red.p);
   }
+  delete ctx->eng_na;
   delete ctx->eng;
   delete ctx;
   return ENT_OK;
@@ -148,10 +177,13 @@ int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h) {
     set_error("invalid tuning (q in {1,2,4}, 1<=m0<=64, 1<=mr<=8, 1<=group_h<=64)");
     return ENT_ERR_ARG;
   }
-  ctx->eng->tuning.q = q;
-  ctx->eng->tuning.m0 = m0;
-  ctx->eng->tuning.mr = mr;
-  ctx->eng->tuning.group_h = group_h;
+  for (ent::Engine* e : {ctx->eng, ctx->eng_na}) {
+    if (!e) continue;
+    e->tuning.q = q;
+    e->tuning.m0 = m0;
+    e->tuning.mr = mr;
+    e->tuning.group_h = group_h;
+  }
   return ENT_OK;
 }
 

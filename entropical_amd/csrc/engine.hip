@@ -62,16 +62,58 @@ Engine::~Engine() {
                     &d_wjobs, &d_wchunks, &d_tchunks, &d_moments, &d_mlist, &d_moments2, &d_mlist2};
   for (GraphCache* c : {&pre_graphs_, &dens_graphs_})
     for (auto& e : c->entries) (void)hipGraphExecDestroy(e.second);
-  for (DevBuf* b : bufs)
-    if (b->p) (void)hipFree(b->p);
+  if (!arena())  // arena buffers belong to the caller's range
+    for (DevBuf* b : bufs)
+      if (b->p) (void)hipFree(b->p);
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
   if (host_stage_) (void)hipHostFree(host_stage_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+int Engine::use_arena(void* base, size_t cap, size_t host_cap) {
+  // the last 64 KiB stay unassigned, so a kernel's read-ahead past the end
+  // of the last buffer stays inside the range
+  constexpr size_t kGuard = 64 * 1024;
+  if (!base || cap <= 16 * kGuard) {
+    set_error("fixed arena: null or too small range");
+    return kErrArg;
+  }
+  ENT_TRY(ensure_host(host_cap));
+  arena_base_ = (char*)base;
+  arena_cap_ = cap - kGuard;
+  use_graph = false;
+  reset_arena();
+  return kOk;
+}
+
+// Forget every buffer of the previous run: this run carves them again from
+// the start of the arena.
+void Engine::reset_arena() {
+  DevBuf* bufs[] = {&d_jobs,  &d_chunks, &d_chunk_first, &d_mm,     &d_flags,        &d_keys[0],
+                    &d_keys[1], &d_idx[0], &d_idx[1],     &d_hist,   &d_scan_scratch, &d_scoord,
+                    &d_tile_cnt, &d_tiles, &d_counter, &d_cellstart, &d_bjobs, &d_bchunks,
+                    &d_bchunk_first, &d_bbase, &d_bkeys[0], &d_bkeys[1], &d_bidx[0], &d_bidx[1],
+                    &d_wjobs, &d_wchunks, &d_tchunks, &d_moments, &d_mlist, &d_moments2, &d_mlist2};
+  for (DevBuf* b : bufs) *b = DevBuf{};
+  arena_off_ = 0;
+  ++generation_;
+}
+
 int Engine::ensure(DevBuf& b, size_t bytes) {
   if (bytes <= b.cap) return kOk;
+  if (arena()) {
+    const size_t off = (arena_off_ + 255) & ~(size_t)255;
+    const size_t want = std::max<size_t>(bytes, 256);
+    if (off > arena_cap_ || want > arena_cap_ - off) {
+      set_error("workspace exceeds the fixed scratch range of " + std::to_string(arena_cap_ >> 20) + " MiB");
+      return kErrUnsupported;
+    }
+    b.p = arena_base_ + off;
+    b.cap = want;
+    arena_off_ = off + want;
+    return kOk;
+  }
   ENT_HIP(hipStreamSynchronize(stream_));
   if (b.p) ENT_HIP(hipFree(b.p));
   b.p = nullptr;
@@ -87,6 +129,10 @@ int Engine::ensure(DevBuf& b, size_t bytes) {
 
 int Engine::ensure_host(size_t bytes) {
   if (bytes <= host_stage_cap_) return kOk;
+  if (arena()) {
+    set_error("host staging exceeds the fixed buffer of " + std::to_string(host_stage_cap_) + " bytes");
+    return kErrUnsupported;
+  }
   ENT_HIP(hipStreamSynchronize(stream_));
   if (host_stage_) ENT_HIP(hipHostFree(host_stage_));
   host_stage_ = nullptr;
@@ -105,6 +151,7 @@ int Engine::densities(const float* X, const std::vector<JobSpec>& jobs, double* 
   }
   ENT_HIP(hipSetDevice(device_));
   last = RunStats{};
+  if (arena()) reset_arena();
   if (jobs.empty()) return kOk;
   for (const JobSpec& j : jobs) {
     if (j.d < 1 || j.d > 3) {

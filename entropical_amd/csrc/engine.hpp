@@ -107,6 +107,16 @@ class Engine {
   int ensure_host(size_t bytes);
   void* host_stage() { return host_stage_; }
 
+  // Fixed-arena mode (for callers that cannot allocate): every workspace
+  // buffer of a run is carved afresh from the device range [base, base + cap)
+  // (256-byte aligned, in a fixed order), the host staging buffer of
+  // host_cap bytes is allocated here, once, and launches are issued without
+  // graph capture.  A run whose workspace does not fit fails with
+  // kErrUnsupported; nothing is allocated or freed after this call.
+  int use_arena(void* base, size_t cap, size_t host_cap);
+  bool arena() const { return arena_base_ != nullptr; }
+  size_t arena_capacity() const { return arena_cap_; }
+
  private:
   int plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, double* dens, int stop);
   int run_brute(const float* X, const std::vector<JobSpec>& jobs, double* dens);
@@ -119,6 +129,10 @@ class Engine {
   };
   template <class F>
   int run_captured(GraphCache& cache, const PlanKey& key, F&& issue);
+
+  void reset_arena();
+  char* arena_base_ = nullptr;
+  size_t arena_cap_ = 0, arena_off_ = 0;
 
   GraphCache pre_graphs_, dens_graphs_;
   uint64_t generation_ = 0;  // bumped on every workspace reallocation

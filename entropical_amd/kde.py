@@ -231,6 +231,22 @@ def cell_sort(coords, n_rows: int, i_cols, h, tau=None, ctx: Optional[Context] =
     return keys, perm, nb, mn, bs
 
 
+def combined_densities_dev(ctx: Context, x_dev: int, n_rows: int, n_cols: int, i_cols: Sequence[int],
+                           h: Sequence[float], out_dev: int, tau: Optional[Sequence[int]] = None,
+                           method: str = "cell", noalloc: bool = False) -> None:
+    """combined_densities over device-resident column-major coordinates into
+    a device array of n_rows - max(tau) doubles (ent_combined_densities_dev).
+    noalloc: the variant that allocates nothing, its workspace being the
+    device's fixed scratch array (ent_combined_densities_dev_noalloc; at most
+    ENT_NOALLOC_MAX_POINTS points)."""
+    n_dim = _check_dims(i_cols, h, tau)
+    ic, hh = _u32(i_cols), _f32(h)
+    tt = _u32(tau) if tau is not None else None
+    fn = load().ent_combined_densities_dev_noalloc if noalloc else load().ent_combined_densities_dev
+    check(fn(ctx.handle, C.c_void_p(x_dev), n_rows, n_cols, _ptr(ic), _ptr(hh),
+             _ptr(tt) if tt is not None else None, n_dim, _METHODS[method], C.c_void_p(out_dev)))
+
+
 def batch_densities_dev(ctx: Context, x_dev: int, jobs: Sequence[dict], dens_dev: int,
                         method: str = "cell") -> None:
     """Run many density jobs over device-resident coordinates (see ent_job)."""

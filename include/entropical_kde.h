@@ -151,6 +151,27 @@ int ent_combined_densities_dev(ent_ctx* ctx, const float* coords_dev, size_t n_r
                                const uint32_t* i_cols, const float* h, const uint32_t* tau,
                                uint32_t n_dim, int method, double* out_dev);
 
+/* Limits of ent_combined_densities_dev_noalloc: the bytes of its fixed
+ * device scratch array, and the most points (n_rows - max(tau)) a call may
+ * have. */
+#define ENT_NOALLOC_SCRATCH_BYTES (128u * 1024u * 1024u)
+#define ENT_NOALLOC_MAX_POINTS 1000000u
+
+/* ent_combined_densities_dev that allocates nothing (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as
+ * serving loops and harnesses that forbid hipMalloc/hipFree.  Same arguments,
+ * results and errors as ent_combined_densities_dev (bit-identical densities
+ * for the same tuning), with coords_dev and out_dev owned by the caller; the
+ * workspace is a fixed device array of ENT_NOALLOC_SCRATCH_BYTES per GPU,
+ * located when the context is created and held until the process ends.
+ * More than ENT_NOALLOC_MAX_POINTS points return ENT_ERR_UNSUPPORTED (as
+ * does a workspace that would not fit).  No device or pinned host memory is
+ * allocated or freed and no graph is captured during the call.  All contexts
+ * on one device share the array: calls that overlap on one GPU race. */
+int ent_combined_densities_dev_noalloc(ent_ctx* ctx, const float* coords_dev, size_t n_rows, size_t n_cols,
+                                       const uint32_t* i_cols, const float* h, const uint32_t* tau,
+                                       uint32_t n_dim, int method, double* out_dev);
+
 /* Many density estimates in one engine run (no reference counterpart: the
  * reference makes one combined_densities call per density, mi.cpp:37-52).
  * X_dev and dens_dev are device pointers; jobs is a host array. */
