@@ -23,7 +23,7 @@ LIB = PKG / "libentropical_kde.so"
 ARCH = os.environ.get("ENT_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["kde_kernels.hip", "engine.hip", "autotune.hip", "capi.hip", "reductions.hip", "drivers.hip",
-               "mlp.hip", "noalloc.hip"]
+               "mlp.hip", "noalloc.hip", "colstats.hip"]
 HOST_SOURCES = ["densities_shim.cpp", "xtc.cpp"]
 
 HIPCC_FLAGS = [

@@ -3,7 +3,9 @@
 //
 // Weights live in FP32 (master copy, updated by Adam) and are rounded to
 // BF16 (round-to-nearest-even) before every forward pass; forward and
-// backward read the BF16 weights and accumulate in FP32.
+// backward read the BF16 weights and accumulate in FP32.  The layer products
+// (forward, and both products of the backward pass) run on the matrix cores
+// (FP32 MFMA, 16 x 16 tiles, one wave each).
 //
 // Layer l maps a_{l} (B x n_l, FP32) to z_{l+1} = a_l W_l + b_l, with
 // a_{l+1} = tanh(z_{l+1}) for hidden layers and the identity for the output.
@@ -34,17 +36,46 @@ __device__ __forceinline__ uint16_t f_to_bf16(float f) {
   return (uint16_t)(u >> 16);
 }
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// One wave's 16 x 16 tile of C = A B (M x N, inner size K) on the matrix
+// cores, FP32 in and out (v_mfma_f32_16x16x4_f32): per step of 4 along K,
+// lane l supplies A(m0 + l % 16, k0 + l / 16) and B(k0 + l / 16, n0 + l % 16);
+// element i of lane l's accumulator is C(m0 + 4 (l / 16) + i, n0 + l % 16).
+// Operands outside the matrices are zero; epi(m, n, c) receives each
+// element of the tile inside C.  The BF16 weights enter as their exact FP32
+// values, so the products are those of the FP32 model with BF16 weights.
+template <class FA, class FB, class FE>
+__device__ __forceinline__ void mfma_tile(int M, int N, int K, int m0, int n0, FA a, FB b, FE epi) {
+  const int l = (int)__lane_id();
+  const int r = l % 16, g = l / 16;
+  floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const int k = k0 + g;
+    const float av = (m0 + r < M && k < K) ? a(m0 + r, k) : 0.0f;
+    const float bv = (k < K && n0 + r < N) ? b(k, n0 + r) : 0.0f;
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + 4 * g + i, n = n0 + r;
+    if (m < M && n < N) epi(m, n, acc[i]);
+  }
+}
+
 // Y[b][o] = act(sum_i A[b][i] * W[i][o] + bias[o]); W is BF16 [n_in][n_out].
-__global__ void k_linear_fwd(const float* __restrict__ A, const uint16_t* __restrict__ W,
-                             const float* __restrict__ bias, float* __restrict__ Y, int B, int n_in,
-                             int n_out, int use_tanh) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
-  if (o >= n_out || b >= B) return;
-  const float* a = A + (size_t)b * n_in;
-  float acc = bias[o];
-  for (int i = 0; i < n_in; ++i) acc = __builtin_fmaf(a[i], bf16_to_f(W[(size_t)i * n_out + o]), acc);
-  Y[(size_t)b * n_out + o] = use_tanh ? tanhf(acc) : acc;
+// Grid: (ceil(n_out / 16), ceil(B / 16)) waves.
+__global__ __launch_bounds__(64) void k_linear_fwd(const float* __restrict__ A, const uint16_t* __restrict__ W,
+                                                   const float* __restrict__ bias, float* __restrict__ Y, int B,
+                                                   int n_in, int n_out, int use_tanh) {
+  mfma_tile(
+      B, n_out, n_in, (int)blockIdx.y * 16, (int)blockIdx.x * 16,
+      [&](int b, int i) { return A[(size_t)b * n_in + i]; },
+      [&](int i, int o) { return bf16_to_f(W[(size_t)i * n_out + o]); },
+      [&](int b, int o, float c) {
+        const float z = c + bias[o];
+        Y[(size_t)b * n_out + o] = use_tanh ? tanhf(z) : z;
+      });
 }
 
 // dZ of the output layer for MSE: dZ[b] = 2 (y[b] - t[b]) / B; loss partial.
@@ -67,35 +98,34 @@ __global__ void k_mse_grad(const float* __restrict__ y, const float* __restrict_
 }
 
 // dA[b][i] = (sum_o dZ[b][o] * W[i][o]) * (1 - A[b][i]^2)   (tanh input layer)
-__global__ void k_linear_bwd_data(const float* __restrict__ dZ, const uint16_t* __restrict__ W,
-                                  const float* __restrict__ A, float* __restrict__ dA, int B, int n_in,
-                                  int n_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
-  if (i >= n_in || b >= B) return;
-  const float* dz = dZ + (size_t)b * n_out;
-  const uint16_t* w = W + (size_t)i * n_out;
-  float acc = 0.0f;
-  for (int o = 0; o < n_out; ++o) acc = __builtin_fmaf(dz[o], bf16_to_f(w[o]), acc);
-  const float a = A[(size_t)b * n_in + i];
-  dA[(size_t)b * n_in + i] = acc * (1.0f - a * a);
+// Grid: (ceil(n_in / 16), ceil(B / 16)) waves.
+__global__ __launch_bounds__(64) void k_linear_bwd_data(const float* __restrict__ dZ,
+                                                        const uint16_t* __restrict__ W, const float* __restrict__ A,
+                                                        float* __restrict__ dA, int B, int n_in, int n_out) {
+  mfma_tile(
+      B, n_in, n_out, (int)blockIdx.y * 16, (int)blockIdx.x * 16,
+      [&](int b, int o) { return dZ[(size_t)b * n_out + o]; },
+      [&](int o, int i) { return bf16_to_f(W[(size_t)i * n_out + o]); },
+      [&](int b, int i, float c) {
+        const float a = A[(size_t)b * n_in + i];
+        dA[(size_t)b * n_in + i] = c * (1.0f - a * a);
+      });
 }
 
-// dW[i][o] = sum_b A[b][i] dZ[b][o];  db[o] = sum_b dZ[b][o]  (FP32)
-__global__ void k_linear_bwd_weight(const float* __restrict__ A, const float* __restrict__ dZ,
-                                    float* __restrict__ dW, float* __restrict__ db, int B, int n_in,
-                                    int n_out) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;  // i == n_in computes the bias gradient
-  if (o >= n_out || i > n_in) return;
-  float acc = 0.0f;
-  if (i < n_in) {
-    for (int b = 0; b < B; ++b) acc = __builtin_fmaf(A[(size_t)b * n_in + i], dZ[(size_t)b * n_out + o], acc);
-    dW[(size_t)i * n_out + o] = acc;
-  } else {
-    for (int b = 0; b < B; ++b) acc += dZ[(size_t)b * n_out + o];
-    db[o] = acc;
-  }
+// dW[i][o] = sum_b A[b][i] dZ[b][o];  db[o] = sum_b dZ[b][o] (row i = n_in
+// of the same product, with A's extra column all ones).
+// Grid: (ceil(n_out / 16), ceil((n_in + 1) / 16)) waves.
+__global__ __launch_bounds__(64) void k_linear_bwd_weight(const float* __restrict__ A, const float* __restrict__ dZ,
+                                                          float* __restrict__ dW, float* __restrict__ db, int B,
+                                                          int n_in, int n_out) {
+  mfma_tile(
+      n_in + 1, n_out, B, (int)blockIdx.y * 16, (int)blockIdx.x * 16,
+      [&](int i, int b) { return i < n_in ? A[(size_t)b * n_in + i] : 1.0f; },
+      [&](int b, int o) { return dZ[(size_t)b * n_out + o]; },
+      [&](int i, int o, float c) {
+        if (i < n_in) dW[(size_t)i * n_out + o] = c;
+        else db[o] = c;
+      });
 }
 
 // Adam on FP32 master weights; refreshes the BF16 copy.
@@ -211,8 +241,8 @@ float* act_ptr(ent_mlp* M, float* base, size_t B, int l) {
 void forward(ent_mlp* M, hipStream_t s, int B) {
   for (int l = 0; l < M->n_layers; ++l) {
     const int ni = M->dims[l], no = M->dims[l + 1];
-    dim3 grid((no + 127) / 128, B);
-    hipLaunchKernelGGL(k_linear_fwd, grid, dim3(128), 0, s, act_ptr(M, M->act, B, l), M->p16 + M->w_off[l],
+    dim3 grid((no + 15) / 16, (B + 15) / 16);
+    hipLaunchKernelGGL(k_linear_fwd, grid, dim3(64), 0, s, act_ptr(M, M->act, B, l), M->p16 + M->w_off[l],
                        M->p + M->b_off[l], act_ptr(M, M->act, B, l + 1), B, ni, no, l + 1 < M->n_layers ? 1 : 0);
   }
 }
@@ -225,10 +255,10 @@ void backward(ent_mlp* M, hipStream_t s, int B, const float* targets) {
   for (int l = L - 1; l >= 0; --l) {
     const int ni = M->dims[l], no = M->dims[l + 1];
     float* dZl = act_ptr(M, M->dact, B, l + 1);
-    hipLaunchKernelGGL(k_linear_bwd_weight, dim3((no + 127) / 128, ni + 1), dim3(128), 0, s,
+    hipLaunchKernelGGL(k_linear_bwd_weight, dim3((no + 15) / 16, (ni + 1 + 15) / 16), dim3(64), 0, s,
                        act_ptr(M, M->act, B, l), dZl, M->g + M->w_off[l], M->g + M->b_off[l], B, ni, no);
     if (l > 0)
-      hipLaunchKernelGGL(k_linear_bwd_data, dim3((ni + 127) / 128, B), dim3(128), 0, s, dZl,
+      hipLaunchKernelGGL(k_linear_bwd_data, dim3((ni + 15) / 16, (B + 15) / 16), dim3(64), 0, s, dZl,
                          M->p16 + M->w_off[l], act_ptr(M, M->act, B, l), act_ptr(M, M->dact, B, l), B, ni, no);
   }
 }

@@ -10,10 +10,13 @@ log-likelihood argmax over 32 candidate multipliers, computed by the GPU KDE
 ``epa_convolution``) remain the fallback and the yardstick.
 
 Training data: seeded synthetic samples with d in 1..3, N log-uniform in
-[1000, 200000], 1-5 Gaussian or Student-t components with random
+[1000, 50000], 1-5 Gaussian or Student-t components with random
 covariances.  The per-feature range of the training set is stored with the
-weights; inputs outside it get the AMISE bandwidth (hestimate) instead of the
-network's.
+weights; inputs whose shape features fall outside it get the AMISE bandwidth
+(hestimate) instead of the network's.  The sample size is not part of that
+check: the rule of thumb carries the N^(-1/7) scaling, and the network
+extrapolates in log10 N beyond the training range (the MI workload's columns
+have N = 200,000).
 """
 from __future__ import annotations
 
@@ -38,7 +41,7 @@ WEIGHTS = Path(__file__).resolve().parent / "learned_h_weights.npz"
 
 
 # ----------------------------------------------------------------- data
-N_MIN, N_MAX = 1000, 200_000
+N_MIN, N_MAX = 1000, 50_000
 
 
 def synth_dataset(rng: np.random.Generator, n_min: int = N_MIN, n_max: int = N_MAX) -> np.ndarray:
@@ -74,19 +77,65 @@ def thumb(x: np.ndarray) -> np.ndarray:
 
 
 _QS = np.array([0.01, 0.05, 0.10, 0.25, 0.50, 0.75, 0.90, 0.95, 0.99])
+N_RANKS = 2 * len(_QS)  # order statistics per dimension (ENT_COLSTATS_RANKS)
+N_HIST = 32             # bins of the z histogram over [-4, 4]
 
 
-def features(x: np.ndarray) -> np.ndarray:
-    """24 sample statistics of a (d, N) sample (scale-free: per-dim z-scores)."""
+def quantile_ranks(n: int):
+    """0-based ranks of the order statistics the quantiles interpolate
+    between, and the interpolation weights, as numpy's 'linear' quantile
+    computes them: virtual index n q + (1 - q) - 1 = q (n - 1)."""
+    vi = n * _QS + (1.0 + _QS * (1.0 - 1.0 - 1.0)) - 1.0
+    lo = np.floor(vi)
+    gamma = vi - lo
+    lo = lo.astype(np.int64).clip(0, n - 1)
+    hi = (lo + 1).clip(0, n - 1)
+    return np.concatenate([lo, hi]).astype(np.uint32), gamma
+
+
+def _lerp(a, b, t):
+    """numpy's quantile interpolation: a + (b - a) t, or b - (b - a)(1 - t)
+    from t = 1/2 on."""
+    d = b - a
+    return np.where(t >= 0.5, b - d * (1.0 - t), a + d * t)
+
+
+def column_stats(x: np.ndarray) -> np.ndarray:
+    """Sufficient statistics of one dimension's sample (1-D float32) for the
+    features, host version of ent_column_stats_dev: mean, population
+    variance, the order statistics at quantile_ranks, and with z the z-score:
+    sum z^3, sum z^4, sum |z|, count |z| > 3, 32-bin histogram of z over
+    [-4, 4].  Layout of one ENT_COLSTATS_WIDTH row."""
     x = np.asarray(x, dtype=np.float64)
-    d, n = x.shape
-    mu = x.mean(axis=1, keepdims=True)
-    sd = x.std(axis=1, keepdims=True)
-    sd[sd == 0] = 1.0
+    n = x.size
+    mu = x.mean()
+    var = x.var()
+    sd = np.sqrt(var) if var > 0 else 1.0
     z = (x - mu) / sd
-    q = np.quantile(z, _QS, axis=1)               # (9, d)
-    skew = (z ** 3).mean(axis=1)
-    kurt = (z ** 4).mean(axis=1) - 3.0
+    ranks, _ = quantile_ranks(n)
+    os_ = np.partition(x, np.unique(ranks))[ranks]
+    hcount, _ = np.histogram(z, bins=N_HIST, range=(-4, 4))
+    return np.concatenate([[mu, var], os_, [(z ** 3).sum(), (z ** 4).sum(), np.abs(z).sum(),
+                                            float(np.count_nonzero(np.abs(z) > 3.0))], hcount.astype(np.float64)])
+
+
+def features_from_stats(st: np.ndarray, n: int, corr: Optional[np.ndarray] = None) -> np.ndarray:
+    """The 24 features of a (d, N) sample from its dimensions' statistics
+    (st: d rows of column_stats layout) and, for d > 1, the correlation
+    matrix of its dimensions."""
+    st = np.atleast_2d(np.asarray(st, dtype=np.float64))
+    d = st.shape[0]
+    mu, var = st[:, 0], st[:, 1]
+    sd = np.sqrt(var)
+    sd[var == 0] = 1.0
+    _, gamma = quantile_ranks(n)
+    nq = len(_QS)
+    zlo = (st[:, 2:2 + nq] - mu[:, None]) / sd[:, None]
+    zhi = (st[:, 2 + nq:2 + 2 * nq] - mu[:, None]) / sd[:, None]
+    q = _lerp(zlo, zhi, gamma[None, :]).T  # (9, d)
+    k = 2 + N_RANKS
+    skew = st[:, k] / n
+    kurt = st[:, k + 1] / n - 3.0
     iqr = q[5] - q[3]
     f = np.zeros(N_FEATURES)
     f[0] = np.log10(n)
@@ -98,22 +147,51 @@ def features(x: np.ndarray) -> np.ndarray:
     f[14] = (q[8] - q[0]).mean()
     f[15] = ((skew ** 2 + 1.0) / (kurt + 3.0)).mean()
     if d > 1:
-        cr = np.corrcoef(z)
-        off = np.abs(cr[~np.eye(d, dtype=bool)])
+        off = np.abs(corr[~np.eye(d, dtype=bool)])
         f[16] = off.mean()
         f[17] = off.max()
-        f[18] = max(np.log(max(np.linalg.det(cr), 1e-12)), -10.0) / 10.0
+        f[18] = max(np.log(max(np.linalg.det(corr), 1e-12)), -10.0) / 10.0
     f[19] = np.minimum(1.0, iqr / 1.349).mean()
-    f[20] = (np.abs(z) > 3.0).mean() * 10.0
-    f[21] = np.abs(z).mean(axis=1).mean()
+    f[20] = st[:, k + 3].sum() / (d * n) * 10.0
+    f[21] = (st[:, k + 2] / n).mean()
     f[22] = np.log2(n ** (-1.0 / (d + 4)) / n ** (-1.0 / 7.0))
     modes = []
     for e in range(d):
-        hcount, _ = np.histogram(z[e], bins=32, range=(-4, 4))
+        hcount = st[e, k + 4:k + 4 + N_HIST]
         hs = np.convolve(hcount, np.ones(3) / 3.0, mode="same")
         modes.append(np.sum((hs[1:-1] > hs[:-2]) & (hs[1:-1] >= hs[2:]) & (hs[1:-1] > 0.01 * hs.max())))
     f[23] = np.mean(modes) / 5.0
     return f.astype(np.float32)
+
+
+def features(x: np.ndarray) -> np.ndarray:
+    """24 sample statistics of a (d, N) sample (scale-free: per-dim z-scores):
+    mean quantiles of the z-scores, skewness, kurtosis, spreads, tail mass,
+    modality, correlation structure for d > 1, and N, d."""
+    x = np.asarray(x, dtype=np.float64)
+    d, n = x.shape
+    st = np.stack([column_stats(x[e]) for e in range(d)])
+    corr = np.corrcoef(x) if d > 1 else None
+    return features_from_stats(st, n, corr)
+
+
+def column_stats_dev(ctx: Context, x_dev: int, n_rows: int, n_cols: int) -> np.ndarray:
+    """column_stats of every column of device-resident column-major X,
+    computed on the device (ent_column_stats_dev): (n_cols, 56) float64."""
+    ranks, _ = quantile_ranks(n_rows)
+    ranks = np.ascontiguousarray(ranks, dtype=np.uint32)
+    st = np.empty((n_cols, 2 + N_RANKS + 4 + N_HIST), np.float64)
+    check(load().ent_column_stats_dev(ctx.handle, C.c_void_p(x_dev), n_rows, n_cols, _ptr(ranks), _ptr(st)))
+    return st
+
+
+def column_features_dev(ctx: Context, x_dev: int, n_rows: int, n_cols: int):
+    """The features of every column (as a 1-D sample) of device-resident
+    column-major X: the per-column statistics on the device, the 24 numbers
+    per column assembled from them here.  Returns (features (n_cols, 24),
+    statistics (n_cols, 56))."""
+    st = column_stats_dev(ctx, x_dev, n_rows, n_cols)
+    return np.stack([features_from_stats(st[c:c + 1], n_rows) for c in range(n_cols)]), st
 
 
 # --------------------------------------------------------------- labels
@@ -271,32 +349,60 @@ class BandwidthModel:
         return self.train_raw(self._norm(F), t, epochs, batch, lr, seed)
 
     def predict_log2r(self, F: np.ndarray) -> np.ndarray:
-        lr = self.predict_raw(self._norm(np.atleast_2d(F)))
+        """log2 of the multiplier of the rule of thumb.  A sample larger (or
+        smaller) than the training range is evaluated at the nearest trained
+        N, and the multiplier then follows the asymptotic scaling of the
+        AMISE-optimal bandwidth, N^(-1/(d+4)) against the rule of thumb's
+        N^(-1/7): feature 22 is that log ratio, so the correction is its
+        change between the two N."""
+        F = np.array(np.atleast_2d(F), dtype=np.float64)
+        lo_n, hi_n = float(self.fmin[0]), float(self.fmax[0])
+        log10n = F[:, 0].copy()
+        clamped = np.clip(log10n, lo_n, hi_n) if np.isfinite(lo_n) and np.isfinite(hi_n) else log10n
+        d = F[:, 1]
+        def f22(l10):
+            n = 10.0 ** l10
+            return np.log2(n ** (-1.0 / (d + 4.0)) / n ** (-1.0 / 7.0))
+        corr = f22(log10n) - f22(clamped)
+        F[:, 0] = clamped
+        F[:, 22] = f22(clamped)
+        lr = self.predict_raw(self._norm(F.astype(np.float32))) + corr
         lo, hi = np.log2(R_GRID[0]), np.log2(R_GRID[-1])
         return np.clip(lr, lo, hi)
 
+    # features that depend on N only (log10 N, the log ratio of the N
+    # exponents): outside the envelope check
+    N_FEATURES_OF_N = (0, 22)
+
     def in_envelope(self, F: np.ndarray, margin: float = 0.05) -> np.ndarray:
-        """True per row of F whose features all lie within the training range
-        (widened by `margin` of each feature's span)."""
+        """True per row of F whose shape features all lie within the training
+        range (widened by `margin` of each feature's span); the features of
+        N alone are not checked."""
         F = np.atleast_2d(np.asarray(F, np.float32))
         span = self.fmax - self.fmin
         span = np.where(np.isfinite(span), span, 0.0)
-        return np.all((F >= self.fmin - margin * span) & (F <= self.fmax + margin * span), axis=1)
+        ok = (F >= self.fmin - margin * span) & (F <= self.fmax + margin * span)
+        ok[:, list(self.N_FEATURES_OF_N)] = True
+        return np.all(ok, axis=1)
 
     def bandwidth(self, x: np.ndarray) -> np.ndarray:
         """Per-dimension bandwidth for a (d, N) sample (no envelope check)."""
         return thumb(x) * 2.0 ** float(self.predict_log2r(features(x))[0])
 
 
-def column_bandwidths(ctx: Context, model: BandwidthModel, x_dev, X: np.ndarray):
-    """Learned bandwidth of every column of X (n_cols, n_rows), device copy
-    x_dev: h = r * thumb with the network's r where the column's features lie
-    in the training envelope, else the AMISE bandwidth (hestimate, computed
-    on the device).  Returns (h float32 (n_cols,), used_network bool)."""
+def column_bandwidths(ctx: Context, model: BandwidthModel, x_dev, X: Optional[np.ndarray] = None):
+    """Learned bandwidth of every column of the device array x_dev (n_cols,
+    n_rows): h = r * thumb, thumb = N^(-1/7) * sd (hestimate.cpp:19-30), with
+    the network's r where the column's features lie in the training
+    envelope, else the AMISE bandwidth (hestimate, computed on the device).
+    The features' statistics and the network run on the device.  (X, the
+    host copy, is not needed.)  Returns (h float32 (n_cols,), used_network
+    bool)."""
     from .kde import hestimate
 
-    F = np.stack([features(X[c: c + 1]) for c in range(X.shape[0])])
-    th = np.array([thumb(X[c: c + 1])[0] for c in range(X.shape[0])])
+    n_cols, n_rows = x_dev.shape
+    F, st = column_features_dev(ctx, x_dev.ptr, n_rows, n_cols)
+    th = (float(n_rows) ** (-1.0 / 7.0)) * np.sqrt(st[:, 1])
     ok = model.in_envelope(F)
     h = (th * 2.0 ** model.predict_log2r(F)).astype(np.float32)
     if not ok.all():

@@ -298,6 +298,20 @@ int ent_ctx_last_driver_ms(ent_ctx* ctx, double* ms);
  * stay available as the fallback and the yardstick).  A multilayer perceptron
  * n_in -> hidden^n_hidden_layers -> 1 with tanh units, BF16 weights and FP32
  * accumulation, trained with minibatch Adam by this library's kernels. */
+/* Per-column sample statistics of the learned bandwidth's features (no
+ * reference counterpart; the rule of thumb it refines is
+ * Hestimate::Thumb::Kernel, hestimate.cpp:19-30), computed on the device
+ * for every column of device-resident column-major X: out[c * WIDTH + ...] =
+ * mean, population variance (FP64), the column's order statistics at the
+ * ENT_COLSTATS_RANKS given 0-based ranks (exact, as float values), and with
+ * z = (x - mean) / sd: sum z^3, sum z^4, sum |z|, the count of |z| > 3 and
+ * the 32-bin histogram of z over [-4, 4] (edges -4 + k/4, last bin closed).
+ * out is host memory of n_cols * ENT_COLSTATS_WIDTH doubles. */
+#define ENT_COLSTATS_RANKS 18
+#define ENT_COLSTATS_WIDTH 56
+int ent_column_stats_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const uint32_t* ranks,
+                         double* out);
+
 typedef struct ent_mlp ent_mlp;
 int ent_mlp_create(ent_ctx* ctx, int n_in, int hidden, int n_hidden_layers, uint64_t seed, ent_mlp** out);
 int ent_mlp_destroy(ent_mlp* mlp);
