@@ -55,7 +55,7 @@ Engine::~Engine() {
   // teardown: errors here have no one to report to
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
-  DevBuf* bufs[] = {&d_jobs,  &d_chunks, &d_chunk_first, &d_mm,     &d_flags,        &d_keys[0],
+  DevBuf* bufs[] = {&d_jobs,  &d_chunks, &d_chunk_first, &d_mm,     &d_keys[0],
                     &d_keys[1], &d_idx[0], &d_idx[1],     &d_hist,   &d_scan_scratch, &d_scoord,
                     &d_tile_cnt, &d_tiles, &d_counter, &d_cellstart, &d_bjobs, &d_bchunks,
                     &d_bchunk_first, &d_bbase, &d_bkeys[0], &d_bkeys[1], &d_bidx[0], &d_bidx[1],
@@ -90,7 +90,7 @@ int Engine::use_arena(void* base, size_t cap, size_t host_cap) {
 // Forget every buffer of the previous run: this run carves them again from
 // the start of the arena.
 void Engine::reset_arena() {
-  DevBuf* bufs[] = {&d_jobs,  &d_chunks, &d_chunk_first, &d_mm,     &d_flags,        &d_keys[0],
+  DevBuf* bufs[] = {&d_jobs,  &d_chunks, &d_chunk_first, &d_mm,     &d_keys[0],
                     &d_keys[1], &d_idx[0], &d_idx[1],     &d_hist,   &d_scan_scratch, &d_scoord,
                     &d_tile_cnt, &d_tiles, &d_counter, &d_cellstart, &d_bjobs, &d_bchunks,
                     &d_bchunk_first, &d_bbase, &d_bkeys[0], &d_bkeys[1], &d_bidx[0], &d_bidx[1],
@@ -252,8 +252,8 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   ENT_TRY(ensure(d_chunk_first, sizeof(uint32_t) * (J + 1)));
   ENT_TRY(ensure(d_wjobs, sizeof(DJob) * NW));
   ENT_TRY(ensure(d_wchunks, sizeof(uint2) * NWC));
-  ENT_TRY(ensure(d_mm, sizeof(uint32_t) * 6 * NW));
-  ENT_TRY(ensure(d_flags, 64));
+  // min/max slots of the windows, then the non-finite flag (one read-back)
+  ENT_TRY(ensure(d_mm, sizeof(uint32_t) * (6 * NW + 16)));
   // host staging: jobs + chunks + chunk_first + windows + minmax readback
   size_t stage_bytes = sizeof(DJob) * J + sizeof(uint2) * NC + sizeof(uint32_t) * (J + 1) +
                        8 + sizeof(DJob) * NW + sizeof(uint2) * NWC + sizeof(uint32_t) * 6 * NW + 64;
@@ -282,12 +282,11 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   ENT_HIP(hipMemcpyAsync(d_wjobs.p, hs_wjobs, sizeof(DJob) * NW, hipMemcpyHostToDevice, stream_));
   ENT_HIP(hipMemcpyAsync(d_wchunks.p, hs_wchunks, sizeof(uint2) * NWC, hipMemcpyHostToDevice, stream_));
   launch_init_minmax(stream_, (uint32_t*)d_mm.p, NW);
-  launch_fill_u32(stream_, (uint32_t*)d_flags.p, 0u, 16);
-  launch_minmax(stream_, (const DJob*)d_wjobs.p, (const uint2*)d_wchunks.p, NWC, X, (uint32_t*)d_mm.p,
-                (uint32_t*)d_flags.p);
+  uint32_t* d_flag = (uint32_t*)d_mm.p + 6 * NW;
+  launch_fill_u32(stream_, d_flag, 0u, 16);
+  launch_minmax(stream_, (const DJob*)d_wjobs.p, (const uint2*)d_wchunks.p, NWC, X, (uint32_t*)d_mm.p, d_flag);
   ENT_HIP(hipGetLastError());
-  ENT_HIP(hipMemcpyAsync(hs_mm, d_mm.p, sizeof(uint32_t) * 6 * NW, hipMemcpyDeviceToHost, stream_));
-  ENT_HIP(hipMemcpyAsync(hs_flags, d_flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+  ENT_HIP(hipMemcpyAsync(hs_mm, d_mm.p, sizeof(uint32_t) * (6 * NW + 1), hipMemcpyDeviceToHost, stream_));
   ENT_HIP(hipStreamSynchronize(stream_));
   if (hs_flags[0] != 0) {
     set_error("non-finite coordinate in density input");

@@ -153,7 +153,7 @@ class Engine {
 
  public:
   // workspace buffers
-  DevBuf d_jobs, d_chunks, d_chunk_first, d_mm, d_flags, d_keys[2], d_idx[2], d_hist,
+  DevBuf d_jobs, d_chunks, d_chunk_first, d_mm, d_keys[2], d_idx[2], d_hist,
       d_scan_scratch, d_scoord, d_tile_cnt, d_tiles, d_counter, d_cellstart;
   // two-stage sort: base (shared dim-0 sort) tables and buffers
   DevBuf d_bjobs, d_bchunks, d_bchunk_first, d_bbase, d_bkeys[2], d_bidx[2];
