@@ -245,7 +245,7 @@ def run_mi(args) -> dict:
     ctx = Context(0)
     dx = DeviceArray(ctx, X)
     tun = mi_autotune(ctx, dx, hs) if not args.no_tune else ctx.get_tuning()
-    log(f"tuning (q, m0, mr, group_h) = {tun}")
+    log(f"tuning (q, m0, mr, group_h, sub) = {tun}")
     sec, kern_ms, st, util = timed_steps(ctx, lambda: mi_matrix(ctx, dx, hs), args.steps, args.warmup)
     mi = mi_matrix(ctx, dx, hs)
     assert np.all(np.isfinite(mi)) and np.allclose(mi, mi.T)
@@ -261,7 +261,7 @@ def run_mi(args) -> dict:
             "workload": "mi_matrix_64col_200000frames_h0.15",
             "jobs": "64 1-D + 2016 2-D densities, one persistent launch",
             "n_cols": n_cols, "n_rows": n_rows, "bandwidth": h, "method": "cell list",
-            "tuning_q_m0_mr_group": list(tun),
+            "tuning_q_m0_mr_group_sub": list(tun),
         },
         **r,
         "lane_utilisation": util,
@@ -347,7 +347,7 @@ def run_te(args) -> dict:
     ctx = Context(0)
     dx = DeviceArray(ctx, X)
     tun = te_autotune(ctx, dx, hs, taus) if not args.no_tune else ctx.get_tuning()
-    log(f"tuning (q, m0, mr, group_h) = {tun}")
+    log(f"tuning (q, m0, mr, group_h, sub) = {tun}")
     sec, kern_ms, st, util = timed_steps(ctx, lambda: te_matrix(ctx, dx, hs, taus), args.steps, args.warmup)
     te = te_matrix(ctx, dx, hs, taus)
     r = rates(st, sec)
@@ -360,7 +360,7 @@ def run_te(args) -> dict:
         "value": r["pairs_evaluated"] / sec / 1e9, "unit": "Gpair-evals/s", "ms_per_step": 1e3 * sec,
         "config": {"workload": "te_32series_100000frames_h0.2_tau1-2-5-10",
                    "jobs": f"{jobs_3d} 3-D + {jobs_2d} 2-D + {jobs_1d} 1-D densities, one persistent launch",
-                   "tuning_q_m0_mr_group": list(tun)},
+                   "tuning_q_m0_mr_group_sub": list(tun)},
         **r, "lane_utilisation": util,
         "dense_equiv_gpair_per_s": dense / sec / 1e9,
         "roofline": roofline(st, kern_ms, "te"),
@@ -425,7 +425,7 @@ def run_uneven(args) -> dict:
     ptr = ctx.alloc(buf.nbytes)
     ctx.h2d(ptr, buf)
     tun = batch_autotune(ctx, ptr, jobs) if not args.no_tune else ctx.get_tuning()
-    log(f"tuning (q, m0, mr, group_h) = {tun}")
+    log(f"tuning (q, m0, mr, group_h, sub) = {tun}")
     sec, kern_ms, st, util = timed_steps(ctx, lambda: batch_entropy(ctx, ptr, jobs), args.steps, args.warmup)
     H = batch_entropy(ctx, ptr, jobs)
     r = rates(st, sec)
@@ -437,7 +437,7 @@ def run_uneven(args) -> dict:
         "metric": "Gpair-evals/s (pairs evaluated one by one; 500 uneven trajectories, densities + entropies)",
         "value": r["pairs_evaluated"] / sec / 1e9, "unit": "Gpair-evals/s", "ms_per_step": 1e3 * sec,
         "config": {"workload": "uneven_500traj_2k-500k_d1-3", "points": int(sum(j["n"] for j in jobs)),
-                   "tuning_q_m0_mr_group": list(tun)},
+                   "tuning_q_m0_mr_group_sub": list(tun)},
         **r, "lane_utilisation": util,
         "dense_equiv_gpair_per_s": dense / sec / 1e9,
         "roofline": roofline(st, kern_ms, "uneven"),
@@ -486,7 +486,7 @@ def run_entropy1d(args) -> dict:
         "metric": "Gpair-evals/s (pairs evaluated one by one; 1-D KDE entropy, N=100k)",
         "value": r["pairs_evaluated"] / sec / 1e9, "unit": "Gpair-evals/s",
         "ms_per_step": 1e3 * sec,
-        "config": {"workload": "entropy1d_gmm3_100000_h0.1", "tuning_q_m0_mr_group": list(tun)},
+        "config": {"workload": "entropy1d_gmm3_100000_h0.1", "tuning_q_m0_mr_group_sub": list(tun)},
         "entropy": H, **r, "lane_utilisation": util,
         "dense_equiv_gpair_per_s": float(n) ** 2 / sec / 1e9, "roofline": roofline(st, kern_ms, "entropy1d"),
     }

@@ -67,7 +67,7 @@ SIGNATURES = {
     "ent_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "ent_ctx_create": (C.c_int, [C.c_int, C.POINTER(P)]),
     "ent_ctx_destroy": (C.c_int, [P]),
-    "ent_ctx_set_tuning": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "ent_ctx_set_tuning": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "ent_ctx_last_stats": (C.c_int, [P, F64P, F64P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "ent_ctx_last_pairs": (C.c_int, [P, C.POINTER(C.c_uint64)]),
     "ent_ctx_last_sort": (C.c_int, [P, C.POINTER(C.c_uint32)]),

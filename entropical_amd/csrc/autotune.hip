@@ -2,8 +2,9 @@
 //
 // Search space: maximum queries per lane q (tile = 64*q queries), dim-0 boxes
 // per bandwidth m0, row boxes per bandwidth mr, tile extent along dim 0 in
-// bandwidths group_h.  Every setting of the grid is timed on a subsample of
-// the batch (after one untimed run, so graph capture and workspace growth are
+// bandwidths group_h, and for batches with 2-D jobs the sub-tile size of
+// their tiles.  Every setting of the grid is timed on a subsample of the
+// batch (after one untimed run, so graph capture and workspace growth are
 // excluded); the four fastest are timed again on the whole batch.
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -35,6 +36,7 @@ std::string signature(const std::vector<JobSpec>& jobs, const std::vector<double
     for (uint32_t e = 0; e < s.d; ++e) lsum[s.d - 1][e] += std::log2(std::max(range_h[3 * j + e], 1e-3));
   }
   std::ostringstream os;
+  os << "v2;";  // version of the search space (entries of other versions never match)
   for (int d = 0; d < 3; ++d) {
     double p = (double)pts[d];
     double r = p > 0 ? std::round(p / std::pow(10.0, std::floor(std::log10(p)) - 1)) *
@@ -99,10 +101,15 @@ int Engine::autotune(const float* X, const std::vector<JobSpec>& jobs, double* d
   if (!tune_cache_loaded_) {
     tune_cache_loaded_ = true;
     if (const char* p = cache_path()) {
+      // one entry per line: signature q m0 mr group_h sub
       std::ifstream f(p);
-      std::string s;
-      Tuning t;
-      while (f >> s >> t.q >> t.m0 >> t.mr >> t.group_h) tune_cache_[s] = t;
+      std::string line;
+      while (std::getline(f, line)) {
+        std::istringstream ls(line);
+        std::string s;
+        Tuning t;
+        if (ls >> s >> t.q >> t.m0 >> t.mr >> t.group_h >> t.sub) tune_cache_[s] = t;
+      }
     }
   }
   auto hit = tune_cache_.find(sig);
@@ -131,26 +138,37 @@ int Engine::autotune(const float* X, const std::vector<JobSpec>& jobs, double* d
   // fastest re-timed on the whole batch: differences of a few percent are
   // below the subsample's run-to-run noise.
   // (group_h = 64 spans the whole range of standardised data: no split)
-  const std::vector<int> qs = {1, 2, 4}, m0s = {8, 16, 32}, mrs = {1, 2}, ghs = {4, 16, 64};
+  std::vector<Tuning> grid;
+  auto add = [&](int q, int m0, int mr, int gh, int sub) {
+    Tuning t;
+    t.q = q;
+    t.m0 = m0;
+    t.mr = mr;
+    t.group_h = gh;
+    t.sub = sub;
+    grid.push_back(t);
+  };
+  for (int q : {1, 2, 4})
+    for (int m0 : {8, 16, 32})
+      for (int mr : {1, 2})
+        for (int gh : {4, 16, 64}) add(q, m0, mr, gh, 0);
+  // 2-D sub-tiles of 16 queries: finer rows pay there (a sub-tile's rows
+  // bound its candidate window), wide dim-0 groups as for whole tiles
+  if (std::any_of(jobs.begin(), jobs.end(), [](const JobSpec& j) { return j.d == 2; }))
+    for (int q : {1, 2})
+      for (int m0 : {16, 32})
+        for (int mr : {2, 3, 4}) add(q, m0, mr, 64, 16);
   std::vector<std::pair<double, Tuning>> scored;
-  for (int q : qs)
-    for (int m0 : m0s)
-      for (int mr : mrs)
-        for (int gh : ghs) {
-          Tuning t;
-          t.q = q;
-          t.m0 = m0;
-          t.mr = mr;
-          t.group_h = gh;
-          double ms = 0.0;
-          int rc = eval(t, 1, &ms, sample);
-          if (skippable(rc)) {
-            (void)hipGetLastError();
-            continue;
-          }
-          if (rc != kOk) return fail(rc);
-          scored.emplace_back(ms, t);
-        }
+  for (const Tuning& t : grid) {
+    double ms = 0.0;
+    int rc = eval(t, 1, &ms, sample);
+    if (skippable(rc)) {
+      (void)hipGetLastError();
+      continue;
+    }
+    if (rc != kOk) return fail(rc);
+    scored.emplace_back(ms, t);
+  }
   if (scored.empty()) {
     set_error("autotune: no tile shape of the grid supports this workload");
     return fail(kErrUnsupported);
@@ -180,7 +198,7 @@ int Engine::autotune(const float* X, const std::vector<JobSpec>& jobs, double* d
   tune_cache_[sig] = cur;
   if (const char* p = cache_path()) {
     std::ofstream f(p, std::ios::app);
-    f << sig << " " << cur.q << " " << cur.m0 << " " << cur.mr << " " << cur.group_h << "\n";
+    f << sig << " " << cur.q << " " << cur.m0 << " " << cur.mr << " " << cur.group_h << " " << cur.sub << "\n";
   }
   if (best_out) *best_out = cur;
   if (best_ms_out) *best_ms_out = cur_ms;

@@ -171,10 +171,11 @@ int ent_ctx_destroy(ent_ctx* ctx) {
   return ENT_OK;
 }
 
-int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h) {
+int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h, int sub) {
   CAPI_TRY(check_ctx(ctx));
-  if (!(q == 1 || q == 2 || q == 4) || m0 < 1 || m0 > 64 || mr < 1 || mr > 8 || group_h < 1 || group_h > 64) {
-    set_error("invalid tuning (q in {1,2,4}, 1<=m0<=64, 1<=mr<=8, 1<=group_h<=64)");
+  if (!(q == 1 || q == 2 || q == 4) || m0 < 1 || m0 > 64 || mr < 1 || mr > 8 || group_h < 1 || group_h > 64 ||
+      !(sub == 0 || sub == 16 || sub == 32)) {
+    set_error("invalid tuning (q in {1,2,4}, 1<=m0<=64, 1<=mr<=8, 1<=group_h<=64, sub in {0,16,32})");
     return ENT_ERR_ARG;
   }
   for (ent::Engine* e : {ctx->eng, ctx->eng_na}) {
@@ -183,21 +184,23 @@ int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h) {
     e->tuning.m0 = m0;
     e->tuning.mr = mr;
     e->tuning.group_h = group_h;
+    e->tuning.sub = sub;
   }
   return ENT_OK;
 }
 
-int ent_ctx_get_tuning(ent_ctx* ctx, int* tuning4) {
+int ent_ctx_get_tuning(ent_ctx* ctx, int* tuning5) {
   CAPI_TRY(check_ctx(ctx));
-  if (!tuning4) {
+  if (!tuning5) {
     set_error("null argument");
     return ENT_ERR_ARG;
   }
   const ent::Tuning& t = ctx->eng->tuning;
-  tuning4[0] = t.q;
-  tuning4[1] = t.m0;
-  tuning4[2] = t.mr;
-  tuning4[3] = t.group_h;
+  tuning5[0] = t.q;
+  tuning5[1] = t.m0;
+  tuning5[2] = t.mr;
+  tuning5[3] = t.group_h;
+  tuning5[4] = t.sub;
   return ENT_OK;
 }
 

@@ -205,7 +205,7 @@ void build_te(size_t N, size_t C, const float* h, const uint32_t* taus, size_t n
 }
 
 // Runs the engine's autotuner on a job set, with ctx->out as scratch output.
-int tune_jobs(ent_ctx* ctx, const float* X, const std::vector<ent::JobSpec>& jobs, int* out4, double* best_ms) {
+int tune_jobs(ent_ctx* ctx, const float* X, const std::vector<ent::JobSpec>& jobs, int* out5, double* best_ms) {
   ent::Engine& E = *ctx->eng;
   uint64_t total = 0;
   for (const auto& j : jobs) total = std::max<uint64_t>(total, j.out_off + j.n);
@@ -213,11 +213,12 @@ int tune_jobs(ent_ctx* ctx, const float* X, const std::vector<ent::JobSpec>& job
   ent::Tuning best;
   double ms = 0.0;
   DRV_TRY(E.autotune(X, jobs, (double*)ctx->out.p, &best, &ms));
-  if (out4) {
-    out4[0] = best.q;
-    out4[1] = best.m0;
-    out4[2] = best.mr;
-    out4[3] = best.group_h;
+  if (out5) {
+    out5[0] = best.q;
+    out5[1] = best.m0;
+    out5[2] = best.mr;
+    out5[3] = best.group_h;
+    out5[4] = best.sub;
   }
   if (best_ms) *best_ms = ms;
   return ENT_OK;
@@ -228,16 +229,16 @@ int tune_jobs(ent_ctx* ctx, const float* X, const std::vector<ent::JobSpec>& job
 extern "C" {
 
 int ent_mi_autotune_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
-                        int* tuning4) {
+                        int* tuning5) {
   DRV_TRY(check_common(ctx, X_dev, n_rows, n_cols, h));
   std::vector<ent::JobSpec> jobs;
   std::vector<ent::RedItem> items;
   build_mi(n_rows, n_cols, h, 0, jobs, items);
-  return tune_jobs(ctx, X_dev, jobs, tuning4, nullptr);
+  return tune_jobs(ctx, X_dev, jobs, tuning5, nullptr);
 }
 
 int ent_te_autotune_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
-                        const uint32_t* taus, size_t n_taus, int* tuning4) {
+                        const uint32_t* taus, size_t n_taus, int* tuning5) {
   DRV_TRY(check_common(ctx, X_dev, n_rows, n_cols, h));
   if (!taus || n_taus == 0) {
     set_error("null argument");
@@ -247,10 +248,10 @@ int ent_te_autotune_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t 
   std::vector<ent::JobSpec> jobs;
   std::vector<ent::RedItem> items;
   build_te(n_rows, n_cols, h, taus, n_taus, 0, jobs, items);
-  return tune_jobs(ctx, X_dev, jobs, tuning4, nullptr);
+  return tune_jobs(ctx, X_dev, jobs, tuning5, nullptr);
 }
 
-int ent_autotune_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs, int* tuning4,
+int ent_autotune_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs, int* tuning5,
                      double* best_ms) {
   if (!ctx || !ctx->eng || !X_dev || !jobs || n_jobs == 0) {
     set_error("null argument or empty job list");
@@ -274,7 +275,7 @@ int ent_autotune_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size
       js[j].h[e] = e < a.n_dim ? a.h[e] : 1.0f;
     }
   }
-  return tune_jobs(ctx, X_dev, js, tuning4, best_ms);
+  return tune_jobs(ctx, X_dev, js, tuning5, best_ms);
 }
 
 int ent_mi_matrix_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,

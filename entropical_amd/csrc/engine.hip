@@ -322,6 +322,13 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
       }
     }
     D.group = (uint32_t)std::max(1, tuning.group_h * tuning.m0);
+    // 2-D jobs with sub-tiles: tiles of up to 64 queries (four of 16 or two
+    // of 32 side by side in one wave)
+    D.tq = (uint32_t)(kWave * tuning.q);
+    if (D.d == 2 && (tuning.sub == 16 || tuning.sub == 32)) {
+      D.tq = kWave;
+      D.flags |= tuning.sub == 16 ? kJobSub16 : kJobSub32;
+    }
     D.cells = cells;
     if (cells + 1 <= kTableCellsPerPoint * (uint64_t)D.n + 4096) {
       D.cbase = table_total;
@@ -366,7 +373,10 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
         while (cf * 2 <= (uint32_t)std::max(1, tuning.m0 / 4)) cf *= 2;
         const uint32_t ncb = (D.nb[0] + cf - 1) / cf;
         const uint64_t cells = (uint64_t)ncb * D.nb[1];
-        if (4 * cells > D.n) continue;  // cells hold 4 points on average at least
+        // cells hold 4 points on average at least (at most 9 B of table per
+        // point); jobs with sub-tiles, whose rows are finer, 1 point
+        const uint64_t per = (D.flags & (kJobSub16 | kJobSub32)) ? 1 : 4;
+        if (per * cells > D.n) continue;
         D.flags |= kJobMoments;
         D.mbase = mom2_total;
         D.mcf = cf;
@@ -386,8 +396,9 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   }
   bool has_mom = mom_total != 0;
   bool has_mom2 = mom2_total != 0;
-  // grid bound of k_cell_moments2 (kBlock cells per block; cells <= points / 4)
-  const uint32_t mom2_blocks = (uint32_t)std::min<uint64_t>(P / (4 * kBlock) + J, 1u << 30);
+  // grid bound of k_cell_moments2 (kBlock cells per block; cells <= points;
+  // the kernel strides over the listed blocks)
+  const uint32_t mom2_blocks = (uint32_t)std::min<uint64_t>(P / kBlock + J, 1u << 30);
   // grid bound of k_box_moments from the plan's shape: the listed boxes are at
   // most the points
   const uint32_t mom_waves = (uint32_t)std::min<uint64_t>(P / kMomentBoxesPerWave + J, 1u << 30);
@@ -611,7 +622,6 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   float* scoord = (float*)d_scoord.p;
 
   sorted_buf = npass & 1;
-  const uint32_t tq = (uint32_t)(kWave * tuning.q);
   uint32_t* tcnt = (uint32_t*)d_tile_cnt.p;
   // stable LSD passes over `np` digits of a chunked batch; the last one of
   // the main batch gathers the coordinates and writes full keys
@@ -650,7 +660,7 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     }
     if (!with_tiles) return;
     const uint2* tch = (const uint2*)d_tchunks.p;
-    launch_cell_starts(stream_, wide, dj, tch, NC, keys[cur], tq, tcnt, (uint32_t*)d_cellstart.p);
+    launch_cell_starts(stream_, wide, dj, tch, NC, keys[cur], tcnt, (uint32_t*)d_cellstart.p);
     if (has_mom)
       launch_box_moments(stream_, dj, (const uint32_t*)d_mlist.p, mom_waves, scoord, (const uint32_t*)d_cellstart.p,
                          (float4*)d_moments.p);
@@ -658,7 +668,7 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
       launch_cell_moments2(stream_, dj, (const uint32_t*)d_mlist2.p, mom2_blocks, scoord,
                            (const uint32_t*)d_cellstart.p, (float*)d_moments2.p);
     launch_exclusive_scan(stream_, tcnt, tcnt, NC, scratch);
-    launch_tile_write(stream_, wide, dj, tch, NC, keys[cur], tq, tcnt, (DTile*)d_tiles.p);
+    launch_tile_write(stream_, wide, dj, tch, NC, keys[cur], tcnt, (DTile*)d_tiles.p);
     launch_tile_ends(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
     if (pack) launch_tile_pack(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
   };
@@ -694,6 +704,7 @@ PlanKey Engine::plan_key(const float* X, const double* dens, uint32_t J, uint32_
   k.m0 = tuning.m0;
   k.mr = tuning.mr;
   k.gh = tuning.group_h;
+  k.sub = tuning.sub;
   k.gen = generation_;
   return k;
 }

@@ -50,11 +50,11 @@ struct PlanKey {
   const void* dens = nullptr;
   uint64_t P = 0, gen = 0, PB = 0;
   uint32_t J = 0, NC = 0, NCB = 0;
-  int npass = 0, raw = 0, q = 0, m0 = 0, mr = 0, gh = 0, grid = 0, variant = 0;
+  int npass = 0, raw = 0, q = 0, m0 = 0, mr = 0, gh = 0, sub = 0, grid = 0, variant = 0;
   bool operator==(const PlanKey& o) const {
     return X == o.X && dens == o.dens && P == o.P && gen == o.gen && PB == o.PB && J == o.J &&
            NC == o.NC && NCB == o.NCB && npass == o.npass && raw == o.raw && q == o.q && m0 == o.m0 &&
-           mr == o.mr && gh == o.gh && grid == o.grid && variant == o.variant;
+           mr == o.mr && gh == o.gh && sub == o.sub && grid == o.grid && variant == o.variant;
   }
 };
 

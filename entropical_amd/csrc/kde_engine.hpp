@@ -35,6 +35,9 @@ constexpr uint32_t kJobMoments = 16u;  // 1-D or 2-D job with a moment table: th
                                        // or cells (2-D) that lie inside every query's
                                        // support are summed from their moments instead
                                        // of pair by pair
+constexpr uint32_t kJobSub16 = 32u;  // 2-D job whose tiles run as four sub-tiles of 16
+                                     // queries, each with its own candidate stream
+constexpr uint32_t kJobSub32 = 64u;  // the same with two sub-tiles of 32 queries
 
 // Device job descriptor.  Points of job j, dim e, index k in [0, n):
 //   X[off[e] + k].
@@ -58,6 +61,8 @@ struct DJob {
   uint64_t mbase;    // first entry of this job's box-moment table (kJobMoments)
   uint32_t mcf;      // 2-D moment cells: dim-0 boxes per cell
   uint32_t mncb;     // 2-D moment cells per row
+  uint32_t tq;       // maximum queries per tile
+  uint32_t pad_;
 };
 
 // Jobs whose cell count exceeds this multiple of their points (sparse grids,
@@ -84,6 +89,9 @@ struct Tuning {
   int m0 = 8;           // dim-0 boxes per bandwidth
   int mr = 1;           // row-dim boxes per bandwidth
   int group_h = 4;      // tile extent bound along dim 0, in bandwidths
+  int sub = 0;          // 2-D jobs: 0 = whole tiles of up to 64*q queries; 16 or 32 = tiles
+                        // of up to 64 queries run as sub-tiles of that many queries, each
+                        // with its own candidate ranges, moment cells and stream
 };
 
 }  // namespace ent

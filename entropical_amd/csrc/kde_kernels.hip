@@ -5,170 +5,11 @@
 // The chunk table (job id, job-local start) is built on the host.
 #include <hip/hip_runtime.h>
 #include <cstdint>
-#include <type_traits>
 #include "kde_engine.hpp"
 #include "kde_kernels.hpp"
-
-// Register budget of the density kernel: at least this many waves per SIMD
-// (HIP's second launch-bounds argument), i.e. at most 512/N VGPRs per lane.
-// 6 keeps the two-queries-per-lane variant at 6 waves without spills (its
-// free allocation took 90 VGPRs, 5 waves); one query per lane is bound at 7
-// waves by its scalar registers either way, and four per lane spills (the
-// autotuner times every setting, so a slower one is not picked).
-#ifndef ENT_DENSITY_MIN_WAVES
-#define ENT_DENSITY_MIN_WAVES 6
-#endif
-
+#include "kde_device.hpp"
 
 namespace ent {
-namespace {
-
-__device__ __forceinline__ int lane_id() { return __lane_id(); }
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Monotone float <-> uint32 map (order preserving) for atomic min/max.
-__device__ __forceinline__ uint32_t f2o(float f) {
-  uint32_t u = __float_as_uint(f);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-
-// Cross-lane steps on the DPP network (no LDS round trip).  Controls of the
-// gfx9 encoding: row_shr:n = 0x110 + n, row_ror:n = 0x120 + n,
-// row_bcast:15 = 0x142, row_bcast:31 = 0x143.
-template <int CTRL, int ROW_MASK = 0xF>
-__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROW_MASK, 0xF, false);
-}
-template <int CTRL, int ROW_MASK = 0xF>
-__device__ __forceinline__ float dppf(float old, float v) {
-  return __uint_as_float(dpp<CTRL, ROW_MASK>(__float_as_uint(old), __float_as_uint(v)));
-}
-
-// Wave minimum / maximum, returned wave-uniform: rotations within each row
-// of 16 lanes (every lane has a source), then rows combined through lanes
-// 15 and 31; lane 63 ends with the result.
-template <bool MAX>
-__device__ __forceinline__ float wave_extreme(float v) {
-  auto op = [](float a, float b) { return MAX ? fmaxf(a, b) : fminf(a, b); };
-  v = op(v, dppf<0x121>(v, v));
-  v = op(v, dppf<0x122>(v, v));
-  v = op(v, dppf<0x124>(v, v));
-  v = op(v, dppf<0x128>(v, v));
-  v = op(v, dppf<0x142, 0xA>(v, v));
-  v = op(v, dppf<0x143, 0xC>(v, v));
-  return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), kWave - 1));
-}
-__device__ __forceinline__ float wave_min(float v) { return wave_extreme<false>(v); }
-__device__ __forceinline__ float wave_max(float v) { return wave_extreme<true>(v); }
-
-// Inclusive wave scan of uint32: shifted adds within rows of 16 (lanes
-// shifted in from outside the row add 0), then row totals carried through
-// lanes 15 and 31.
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  v += dpp<0x111>(0u, v);
-  v += dpp<0x112>(0u, v);
-  v += dpp<0x114>(0u, v);
-  v += dpp<0x118>(0u, v);
-  v += dpp<0x142, 0xA>(0u, v);
-  v += dpp<0x143, 0xC>(0u, v);
-  return v;
-}
-
-// Reductions and scans over groups of W = 16 or 32 consecutive lanes (the
-// sub-tiles of a packed tile); every lane of a group gets its group's
-// result.  Rotations within rows of 16, then for W = 32 the row pair is
-// combined across lane distance 16.
-template <int W, typename Op>
-__device__ __forceinline__ uint32_t sub_reduce(uint32_t v, Op op) {
-  static_assert(W == 16 || W == 32, "sub-group width");
-  v = op(v, dpp<0x121>(v, v));
-  v = op(v, dpp<0x122>(v, v));
-  v = op(v, dpp<0x124>(v, v));
-  v = op(v, dpp<0x128>(v, v));
-  if constexpr (W == 32) v = op(v, (uint32_t)__shfl_xor((int)v, 16, kWave));
-  return v;
-}
-template <int W>
-__device__ __forceinline__ float sub_min(float v) {
-  return __uint_as_float(sub_reduce<W>(__float_as_uint(v), [](uint32_t a, uint32_t b) {
-    return __float_as_uint(fminf(__uint_as_float(a), __uint_as_float(b)));
-  }));
-}
-template <int W>
-__device__ __forceinline__ float sub_max(float v) {
-  return __uint_as_float(sub_reduce<W>(__float_as_uint(v), [](uint32_t a, uint32_t b) {
-    return __float_as_uint(fmaxf(__uint_as_float(a), __uint_as_float(b)));
-  }));
-}
-template <int W>
-__device__ __forceinline__ uint32_t sub_sum(uint32_t v) {
-  return sub_reduce<W>(v, [](uint32_t a, uint32_t b) { return a + b; });
-}
-// inclusive scan within each group of W lanes
-template <int W>
-__device__ __forceinline__ uint32_t sub_incl_scan(uint32_t v) {
-  v += dpp<0x111>(0u, v);
-  v += dpp<0x112>(0u, v);
-  v += dpp<0x114>(0u, v);
-  v += dpp<0x118>(0u, v);
-  if constexpr (W == 32) v += dpp<0x142, 0xA>(0u, v);
-  return v;
-}
-// wave maximum of uint32, wave-uniform
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-  auto op = [](uint32_t a, uint32_t b) { return a > b ? a : b; };
-  v = op(v, dpp<0x121>(v, v));
-  v = op(v, dpp<0x122>(v, v));
-  v = op(v, dpp<0x124>(v, v));
-  v = op(v, dpp<0x128>(v, v));
-  v = op(v, dpp<0x142, 0xA>(v, v));
-  v = op(v, dpp<0x143, 0xC>(v, v));
-  return __builtin_amdgcn_readlane(v, kWave - 1);
-}
-
-// Workgroups are dispatched round-robin over the XCDs (workgroup b runs on
-// XCD b % 8).  This bijection hands each XCD a contiguous share of the
-// workgroup-indexed work list, so the few jobs one XCD works on at a time
-// keep their coordinate columns in that XCD's own L2 instead of every XCD
-// touching every active job.
-constexpr uint32_t kXcds = 8;
-__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t G) {
-  const uint32_t q = G / kXcds, r = G % kXcds, x = b % kXcds;
-  return x * q + (x < r ? x : r) + b / kXcds;
-}
-
-__device__ __forceinline__ uint64_t lanemask_lt() {
-  const int l = lane_id();
-  return l == 0 ? 0ull : (~0ull >> (64 - l));
-}
-
-// Box of x along one dim (boxedsearch.cpp:30-31 arithmetic): floor of the
-// correctly rounded fl(fl(x - mn) / bs), clamped to [0, nb-1].
-__device__ __forceinline__ uint32_t box_of(float x, float mn, float bs, uint32_t nb) {
-  float q = __fdiv_rn(__fsub_rn(x, mn), bs);
-  float f = floorf(q);
-  if (!(f > 0.0f)) return 0u;  // also maps NaN to 0
-  if (f >= (float)nb) return nb - 1u;
-  uint32_t b = (uint32_t)f;
-  return b > nb - 1u ? nb - 1u : b;
-}
-
-__device__ __forceinline__ uint64_t cell_key(const DJob& J, const float* v) {
-  uint64_t k = box_of(v[0], J.mn[0], J.bs[0], J.nb[0]);
-  if (J.d >= 2) {
-    uint64_t r = box_of(v[1], J.mn[1], J.bs[1], J.nb[1]);
-    if (J.d == 3) r += (uint64_t)box_of(v[2], J.mn[2], J.bs[2], J.nb[2]) * J.nb[1];
-    k += r * J.nb[0];
-  }
-  return k;
-}
-
-}  // namespace
 
 // ------------------------------------------------------------------ init
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
@@ -847,8 +688,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_moments2(const DJob* __restrict
 // have at most tq queries and span at most `group` dim-0 boxes.
 namespace {
 template <typename KeyT>
-__device__ __forceinline__ bool tile_start(const DJob& J, KeyT a, KeyT b, uint32_t p, uint32_t tq) {
-  if (p == 0 || (p % tq) == 0) return true;
+__device__ __forceinline__ bool tile_start(const DJob& J, KeyT a, KeyT b, uint32_t p) {
+  if (p == 0 || (p % J.tq) == 0) return true;
   if (J.flags & kJobBrute) return false;
   // key width arithmetic (32-bit division for 32-bit keys)
   const KeyT nb0 = (KeyT)J.nb[0];
@@ -859,8 +700,7 @@ __device__ __forceinline__ bool tile_start(const DJob& J, KeyT a, KeyT b, uint32
 static_assert(kChunkItems <= 32, "tile-start flags are one 32-bit word per lane");
 // Tile-start flags of a chunk: bit i = point y + 64 i + l starts a tile.
 template <typename KeyT>
-__device__ __forceinline__ uint32_t chunk_tile_starts(const DJob& J, uint32_t y, const KeyT* keys,
-                                                      uint32_t tq, int l) {
+__device__ __forceinline__ uint32_t chunk_tile_starts(const DJob& J, uint32_t y, const KeyT* keys, int l) {
   constexpr int kHalf = kChunkItems / 2;
   uint32_t bits = 0;
 #pragma unroll
@@ -870,7 +710,7 @@ __device__ __forceinline__ uint32_t chunk_tile_starts(const DJob& J, uint32_t y,
 #pragma unroll
     for (int i = 0; i < kHalf; ++i) {
       const uint32_t p = y + (uint32_t)(h * kHalf + i) * kWave + l;
-      if (p < J.n && tile_start(J, prev[i], cur[i], p, tq)) bits |= 1u << (h * kHalf + i);
+      if (p < J.n && tile_start(J, prev[i], cur[i], p)) bits |= 1u << (h * kHalf + i);
     }
   }
   return bits;
@@ -884,7 +724,7 @@ template <typename KeyT>
 __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__ jobs,
                                                         const uint2* __restrict__ chunks,
                                                         uint32_t n_chunks, const KeyT* __restrict__ keys,
-                                                        uint32_t tq, uint32_t* __restrict__ tile_cnt,
+                                                        uint32_t* __restrict__ tile_cnt,
                                                         uint32_t* __restrict__ start) {
   const uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (c >= n_chunks) return;
@@ -912,7 +752,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
       const uint32_t p = p0 + (uint32_t)l;
       uint64_t first = 0, cnt = 0;
       if (p < J.n) {
-        if (tile_start(J, prevk[ii], curk[ii], p, tq)) ++tiles;
+        if (tile_start(J, prevk[ii], curk[ii], p)) ++tiles;
         if (tab) {
           const uint64_t cur = curk[ii];
           first = p == 0 ? 0 : (uint64_t)prevk[ii] + 1;
@@ -949,7 +789,6 @@ __global__ __launch_bounds__(kBlock) void k_tile_write(const DJob* __restrict__ 
                                                        const uint2* __restrict__ chunks,
                                                        uint32_t n_chunks,
                                                        const KeyT* __restrict__ keys,
-                                                       uint32_t tq,
                                                        const uint32_t* __restrict__ offs,
                                                        DTile* __restrict__ tiles) {
   const uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
@@ -959,7 +798,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_write(const DJob* __restrict__ 
   const int l = lane_id();
   const uint64_t lt = lanemask_lt();
   uint32_t o = offs[c];
-  const uint32_t bits = chunk_tile_starts(J, ch.y, keys, tq, l);
+  const uint32_t bits = chunk_tile_starts(J, ch.y, keys, l);
 #pragma unroll
   for (int i = 0; i < kChunkItems; ++i) {
     const uint32_t p = ch.y + i * kWave + l;
@@ -1044,788 +883,6 @@ __global__ void k_tile_pack(const DJob* __restrict__ jobs, DTile* __restrict__ t
         ++a;
       }
     }
-  }
-}
-
-// ---------------------------------------------------------- density core
-namespace {
-
-constexpr float kSentinel = 1.0e18f;
-
-__device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
-
-// First position in a[0..n) whose key is >= v (keys compared as uint64).
-template <typename KeyT>
-__device__ __forceinline__ uint32_t lower_bound_key(const KeyT* a, uint32_t n, uint64_t v) {
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if ((uint64_t)a[mid] < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// Tiles whose queries span more than this many bandwidths along dim 0 (sparse
-// stretches; tiles stay inside one dim-0 group, up to 64 bandwidths wide)
-// evaluate u = (x_q - x_c) / h from the raw coordinates.  Narrower tiles use
-// coordinates scaled about the centre of their bounding box, s = (x - o) / h,
-// and u = s_q - s_c: the FP32 rounding of s is about 1.2e-7 |s|, so with
-// |s| at most 17 (16 bandwidths from the centre plus the support) u stays
-// within about 4e-6 of its exact value.  (With the tile's first query as the
-// origin and 64-bandwidth tiles, a sparse query's density was off by 1.6e-5.)
-constexpr float kLocalSpan = 32.0f;
-
-struct WaveLds {
-  float4 cand[kWave];        // staged candidates (local coordinates)
-  uint2 rng[kWave];          // packed tiles: candidate ranges (first flattened index, first position)
-  float4 cell[2][kWave];     // 2-D moment cells (with cand): 9 moments and the cell origin
-};
-
-// Conservative box range [lo, hi] covering [a - h, b + h] along one dim.
-__device__ __forceinline__ void box_range(float a, float b, float h, float mn, float bs, uint32_t nb,
-                                          uint32_t& lo, uint32_t& hi) {
-  float marg_a = (fabsf(a) + h) * 1.0e-6f;
-  float marg_b = (fabsf(b) + h) * 1.0e-6f;
-  lo = box_of(a - h - marg_a, mn, bs, nb);
-  hi = box_of(b + h + marg_b, mn, bs, nb);
-}
-
-template <int D>
-__device__ __forceinline__ void stage_store(WaveLds& L, int l, const float* s) {
-  float* base = reinterpret_cast<float*>(L.cand);
-  if constexpr (D == 1) base[l] = s[0];
-  else if constexpr (D == 2) {
-    base[2 * l] = s[0];
-    base[2 * l + 1] = s[1];
-  } else {
-    L.cand[l] = make_float4(s[0], s[1], s[2], 0.0f);
-  }
-}
-
-__device__ __forceinline__ void kahan_add(float& S, float& C, float x) {
-  float y = x - C;
-  float t = S + y;
-  C = (t - S) - y;
-  S = t;
-}
-
-// Accumulate one staged chunk of 64 candidates into the Q queries of a lane.
-// RAW: stage holds raw coordinates, u = (xq - xc) * hinv (plain path).
-// else: stage holds (x - o) * hinv, u = sq - sc.
-// NC: candidates staged at c4 (64 for a whole chunk, 16 or 32 for the
-// sub-tile of a packed tile, whose lanes read their own sub-tile's stage).
-template <int D, int Q, bool RAW, bool TRI = false, int NC = kWave>
-__device__ __forceinline__ void accumulate_chunk(const float4* c4, const float (&sq)[3][Q],
-                                                 const float (&hinv)[3], float (&S)[Q],
-                                                 float (&C)[Q]) {
-  float part[Q];
-#pragma unroll
-  for (int q = 0; q < Q; ++q) part[q] = 0.0f;
-  constexpr int kPer4 = (D == 1) ? 4 : (D == 2 ? 2 : 1);
-  constexpr int kReads = NC / kPer4;  // float4 LDS reads per chunk
-
-  // the candidates of one float4 read against the lane's queries
-  auto consume = [&](const float4 v, const int g) {
-    float cs[4][3];
-    if constexpr (D == 1) {
-      cs[0][0] = v.x; cs[1][0] = v.y; cs[2][0] = v.z; cs[3][0] = v.w;
-    } else if constexpr (D == 2) {
-      cs[0][0] = v.x; cs[0][1] = v.y; cs[1][0] = v.z; cs[1][1] = v.w;
-    } else {
-      cs[0][0] = v.x; cs[0][1] = v.y; cs[0][2] = v.z;
-    }
-    if constexpr (D == 1) {
-      // four candidates per read: the terms are summed as a small tree so
-      // the accumulator sees one dependent add per four candidates (a
-      // serial chain sub -> fma -> add per candidate leaves the SIMD idle
-      // between dependent instructions)
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        float w4[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          float uu = sq[0][q] - cs[c][0];
-          if constexpr (RAW) uu = uu * hinv[0];
-          if constexpr (TRI) w4[c] = clamp01(1.0f - fabsf(uu));
-          else w4[c] = clamp01(__builtin_fmaf(-uu, uu, 1.0f));
-        }
-        part[q] += (w4[0] + w4[1]) + (w4[2] + w4[3]);
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < kPer4; ++c) {
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-          float w[3];
-#pragma unroll
-          for (int e = 0; e < D; ++e) {
-            float uu = sq[e][q] - cs[c][e];
-            if constexpr (RAW) uu = uu * hinv[e];
-            w[e] = clamp01(__builtin_fmaf(-uu, uu, 1.0f));
-          }
-          if constexpr (D == 2) part[q] = __builtin_fmaf(w[0], w[1], part[q]);
-          else part[q] = __builtin_fmaf(w[0] * w[1], w[2], part[q]);
-        }
-      }
-    }
-    // compensated merge every 32 candidates (and at the end of the stage)
-    if (((g + 1) * kPer4) % 32 == 0 || g + 1 == kReads) {
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        kahan_add(S[q], C[q], part[q]);
-        part[q] = 0.0f;
-      }
-    }
-  };
-
-#pragma unroll
-  for (int g = 0; g < kReads; ++g) consume(c4[g], g);
-}
-
-// Moment boxes [b0, b0 + nbx) of a 1-D tile added into the lanes' queries:
-// 64 boxes at a time, lane k stages box k's origin and moments in LDS, then
-// every lane adds c + d (B - A d), d = xq - r, per box (padding boxes are
-// zero), with a compensated merge every 8 boxes.
-template <int Q>
-__device__ __forceinline__ void moment_boxes(const DJob& J, const float4* __restrict__ mom, uint32_t b0,
-                                             uint32_t nbx, const float (&xq)[3][Q], float (&S)[Q],
-                                             float (&C)[Q], WaveLds& L) {
-  const int l = lane_id();
-  for (uint32_t base = 0; base < nbx; base += kWave) {
-    const uint32_t cnt = min(nbx - base, (uint32_t)kWave);
-    float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
-    float r = 0.f;
-    if ((uint32_t)l < cnt) {
-      const uint32_t b = b0 + base + (uint32_t)l;
-      m = mom[J.mbase + b];
-      r = fmaf((float)b, J.bs[0], J.mn[0]);
-    }
-    wave_sync();
-    L.cand[l] = make_float4(r, m.x, m.y, m.z);
-    wave_sync();
-    const uint32_t cnt8 = (cnt + 7u) & ~7u;
-    for (uint32_t k = 0; k < cnt8; k += 8) {
-      float part[Q];
-#pragma unroll
-      for (int q = 0; q < Q; ++q) part[q] = 0.0f;
-#pragma unroll 2
-      for (int i = 0; i < 8; ++i) {
-        const float4 v = L.cand[k + i];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-          const float d = xq[0][q] - v.x;
-          const float t = __builtin_fmaf(-v.y, d, v.z);
-          part[q] += __builtin_fmaf(t, d, v.w);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < Q; ++q) kahan_add(S[q], C[q], part[q]);
-    }
-  }
-}
-
-// Moment cells of a 2-D tile (rows [r0, r0 + nr) x cells [c0, c0 + nc) of
-// each row) added into the lanes' queries: 64 cells at a time, lane k stages
-// cell k's moments N and origin in LDS, then every lane adds
-// sum_q b_q sum_p a_p N[3p + q] per cell (padding cells are zero), with a
-// compensated merge every 8 cells.
-template <int Q>
-__device__ __forceinline__ void cell_moments(const DJob& J, const float* __restrict__ mom2, uint32_t r0,
-                                             uint32_t nr, uint32_t c0, uint32_t nc, const float (&xq)[3][Q],
-                                             float (&S)[Q], float (&C)[Q], WaveLds& L) {
-  const int l = lane_id();
-  const uint32_t cells = nr * nc;
-  const float k0 = J.hinv[0] * J.hinv[0], k1 = J.hinv[1] * J.hinv[1];
-  for (uint32_t base = 0; base < cells; base += kWave) {
-    const uint32_t cnt = min(cells - base, (uint32_t)kWave);
-    float N[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    float o0 = 0.f, o1 = 0.f;
-    if ((uint32_t)l < cnt) {
-      const uint32_t k = base + (uint32_t)l;
-      const uint32_t ri = k / nc;
-      const uint32_t row = r0 + ri, cb = c0 + (k - ri * nc);
-      const float* src = mom2 + (J.mbase + (uint64_t)row * J.mncb + cb) * 9;
-#pragma unroll
-      for (int i = 0; i < 9; ++i) N[i] = src[i];
-      o0 = fmaf((float)(cb * J.mcf), J.bs[0], J.mn[0]);
-      o1 = fmaf((float)row, J.bs[1], J.mn[1]);
-    }
-    wave_sync();
-    L.cand[l] = make_float4(N[0], N[1], N[2], N[3]);
-    L.cell[0][l] = make_float4(N[4], N[5], N[6], N[7]);
-    L.cell[1][l] = make_float4(N[8], o0, o1, 0.f);
-    wave_sync();
-    const uint32_t cnt8 = (cnt + 7u) & ~7u;
-    for (uint32_t k = 0; k < cnt8; k += 8) {
-      float part[Q];
-#pragma unroll
-      for (int q = 0; q < Q; ++q) part[q] = 0.0f;
-      // one cell per step: unrolled, the LDS reads of several cells would be
-      // held in registers, raising the kernel's register count (and so
-      // lowering its occupancy)
-#pragma unroll 1
-      for (int i = 0; i < 8; ++i) {
-        // a = (N00, N01, N02, N10), b = (N11, N12, N20, N21), c = (N22, r0, r1, -)
-        const float4 a = L.cand[k + i], b = L.cell[0][k + i], c = L.cell[1][k + i];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-          const float d0 = xq[0][q] - c.y, d1 = xq[1][q] - c.z;
-          const float a0 = __builtin_fmaf(-(k0 * d0), d0, 1.0f);
-          const float b0 = __builtin_fmaf(-(k1 * d1), d1, 1.0f);
-          const float w0 = __builtin_fmaf(a0, a.x, __builtin_fmaf(d0, a.w, b.z));
-          const float w1 = __builtin_fmaf(a0, a.y, __builtin_fmaf(d0, b.x, b.w));
-          const float w2 = __builtin_fmaf(a0, a.z, __builtin_fmaf(d0, b.y, c.x));
-          part[q] += __builtin_fmaf(b0, w0, __builtin_fmaf(d1, w1, w2));
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < Q; ++q) kahan_add(S[q], C[q], part[q]);
-    }
-  }
-}
-
-// Densities of one tile's queries; returns the tile's pair-by-pair candidate
-// count (wave-uniform).  mstat[0..1] (jobs with a moment table): candidates
-// summed from moments and the number of those boxes or cells; mstat[2]: 1
-// when the tile's pair terms were taken from raw coordinates (a wide tile).
-template <int D, int Q, bool RAW, typename KeyT>
-__device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __restrict__ X,
-                                 const float* __restrict__ scoord, const KeyT* __restrict__ keys,
-                                 const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ idx,
-                                 const float4* __restrict__ mom, const float* __restrict__ mom2,
-                                 double* __restrict__ dens, WaveLds& L, uint32_t (&mstat)[3]) {
-  const int l = lane_id();
-  const uint32_t n = J.n;
-  const float* src[3];
-#pragma unroll
-  for (int e = 0; e < D; ++e)
-    src[e] = RAW ? (X + J.off[e]) : (scoord + 3 * J.pbase + (uint64_t)e * n);
-  float hinv[3] = {J.hinv[0], J.hinv[1], J.hinv[2]};
-  const bool tri = (J.flags & kJobTri) != 0;
-
-  // queries, and their output slots (loaded now so the tile's end does not
-  // wait for another round trip)
-  float xq[3][Q];
-  bool qv[Q];
-  uint32_t slot[Q];
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    uint32_t p = T.p0 + q * kWave + l;
-    qv[q] = p < T.p1;
-    uint32_t pp = qv[q] ? p : T.p0;
-#pragma unroll
-    for (int e = 0; e < D; ++e) xq[e][q] = src[e][pp];
-    slot[q] = RAW ? pp : idx[J.pbase + pp];
-  }
-
-  float o[3] = {0.f, 0.f, 0.f};
-  bool exact = false;  // raw-coordinate pair terms (a wide tile)
-  uint32_t n_ranges = 1;
-  uint32_t my_lo = 0, my_len = 0;
-  uint32_t m_lo = 0, m_n = 0;  // 1-D: boxes summed from the moment table
-  uint32_t c_r0 = 0, c_nr = 0, c_c0 = 0, c_nc = 0, c_pts = 0;  // 2-D: moment cells and their points
-  if constexpr (RAW) {
-    if (l == 0) { my_lo = 0; my_len = n; }
-  } else {
-    // bounding box of the tile's queries; its centre is the origin
-    float bmin[3], bmax[3];
-#pragma unroll
-    for (int e = 0; e < D; ++e) {
-      float mn = xq[e][0], mx = xq[e][0];
-#pragma unroll
-      for (int q = 1; q < Q; ++q) {
-        mn = fminf(mn, xq[e][q]);
-        mx = fmaxf(mx, xq[e][q]);
-      }
-      bmin[e] = wave_min(mn);
-      bmax[e] = wave_max(mx);
-      o[e] = 0.5f * (bmin[e] + bmax[e]);
-    }
-    exact = (bmax[0] - bmin[0]) * J.hinv[0] > kLocalSpan;
-    uint32_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-#pragma unroll
-    for (int e = 0; e < D; ++e) box_range(bmin[e], bmax[e], J.h[e], J.mn[e], J.bs[e], J.nb[e], lo[e], hi[e]);
-    if constexpr (D == 1) {
-      if (J.flags & kJobMoments) {
-        // Boxes [ilo, ihi] hold only points x with bmax - h < x < bmin + h
-        // (box_of is monotone and the margin covers the rounding of the box
-        // arithmetic), i.e. inside the support of every query of the tile;
-        // they are summed from their moments, the boxes at either end of the
-        // candidate window pair by pair.
-        const float h = J.h[0];
-        const float marg = (fabsf(bmin[0]) + fabsf(bmax[0]) + fabsf(J.mn[0]) + h) * 1.0e-5f;
-        const uint32_t ilo = box_of(bmax[0] - h + marg, J.mn[0], J.bs[0], J.nb[0]) + 1u;
-        const uint32_t ihi1 = box_of(bmin[0] + h - marg, J.mn[0], J.bs[0], J.nb[0]);
-        if (ilo < ihi1) {
-          m_lo = ilo;
-          m_n = ihi1 - ilo;
-        }
-      }
-    }
-    uint32_t n1 = (D >= 2) ? (hi[1] - lo[1] + 1) : 1u;
-    uint32_t n2 = (D == 3) ? (hi[2] - lo[2] + 1) : 1u;
-    n_ranges = n1 * n2;
-    if (n_ranges > (uint32_t)kMaxRanges) n_ranges = kMaxRanges + 1;  // signals the slow path
-    if (m_n != 0) {
-      // two pair-by-pair ranges around the moment boxes (the job has a table)
-      n_ranges = 2;
-      if (l < 2) {
-        const uint32_t* st = cstart + J.cbase;
-        my_lo = st[l == 0 ? lo[0] : m_lo + m_n];
-        my_len = st[l == 0 ? m_lo : hi[0] + 1] - my_lo;
-      }
-    } else if (n_ranges <= (uint32_t)kMaxRanges && (uint32_t)l < n_ranges) {
-      uint32_t r1 = (D >= 2) ? lo[1] + (uint32_t)l % n1 : 0u;
-      uint32_t r2 = (D == 3) ? lo[2] + (uint32_t)l / n1 : 0u;
-      uint64_t row = (uint64_t)r2 * J.nb[1] + r1;
-      uint64_t klo = row * J.nb[0] + lo[0];
-      uint64_t khi = row * J.nb[0] + hi[0];
-      uint32_t a, b;
-      if (J.cbase != kNoTable) {
-        const uint32_t* st = cstart + J.cbase;
-        a = st[klo];
-        b = st[khi + 1];
-      } else {
-        const KeyT* kk = keys + J.pbase;
-        a = lower_bound_key(kk, n, klo);
-        b = a + lower_bound_key(kk + a, n - a, khi + 1);
-      }
-      my_lo = a;
-      my_len = b - a;
-    }
-    if constexpr (D == 2) {
-      if ((J.flags & kJobMoments) && n_ranges <= (uint32_t)kMaxRanges / 2) {
-        // Cells (row, cb) of rows [jlo, jhi1) and dim-0 cells [clo, chi1)
-        // hold only points inside the support of every query (as for 1-D
-        // boxes, per dim).  Used when they hold at least 4 points per cell
-        // on average; the interior rows' pair-by-pair candidates become two
-        // ranges, left and right of the cells (the right ones on lanes
-        // n1 .. n1 + nir - 1).
-        const uint32_t cf = J.mcf;
-        const float h0 = J.h[0], h1 = J.h[1];
-        const float mg0 = (fabsf(bmin[0]) + fabsf(bmax[0]) + fabsf(J.mn[0]) + h0) * 1.0e-5f;
-        const float mg1 = (fabsf(bmin[1]) + fabsf(bmax[1]) + fabsf(J.mn[1]) + h1) * 1.0e-5f;
-        const uint32_t ilo = box_of(bmax[0] - h0 + mg0, J.mn[0], J.bs[0], J.nb[0]) + 1u;
-        const uint32_t ihi1 = box_of(bmin[0] + h0 - mg0, J.mn[0], J.bs[0], J.nb[0]);
-        const uint32_t clo = (ilo + cf - 1u) / cf, chi1 = ihi1 / cf;
-        const uint32_t jlo = max(box_of(bmax[1] - h1 + mg1, J.mn[1], J.bs[1], J.nb[1]) + 1u, lo[1]);
-        const uint32_t jhi1 = min(box_of(bmin[1] + h1 - mg1, J.mn[1], J.bs[1], J.nb[1]), hi[1] + 1u);
-        if (clo < chi1 && jlo < jhi1) {
-          const uint32_t nir = jhi1 - jlo, nc = chi1 - clo;
-          const uint32_t r = lo[1] + (uint32_t)l;
-          const bool inr = (uint32_t)l < n_ranges && r >= jlo && r < jhi1;
-          uint32_t m1 = 0, m2 = 0;
-          if (inr) {
-            const uint32_t* st = cstart + J.cbase + (uint64_t)r * J.nb[0];
-            m1 = st[clo * cf];
-            m2 = st[chi1 * cf];
-          }
-          const uint32_t pts = __builtin_amdgcn_readlane(wave_incl_scan(m2 - m1), kWave - 1);
-          if (pts >= 4u * nir * nc) {
-            const uint32_t end = my_lo + my_len;
-            const int src = ((int)(jlo - lo[1]) + l - (int)n_ranges) & (kWave - 1);
-            const uint32_t s_m2 = __shfl(m2, src);
-            const uint32_t s_end = __shfl(end, src);
-            if (inr) my_len = m1 - my_lo;
-            if ((uint32_t)l >= n_ranges && (uint32_t)l < n_ranges + nir) {
-              my_lo = s_m2;
-              my_len = s_end - s_m2;
-            }
-            n_ranges += nir;
-            c_r0 = jlo;
-            c_nr = nir;
-            c_c0 = clo;
-            c_nc = nc;
-            c_pts = pts;
-          }
-        }
-      }
-    }
-    if (n_ranges > (uint32_t)kMaxRanges) {
-      // More candidate rows than lanes: fall back to one range spanning every
-      // row between the first and last candidate row (a superset).
-      uint64_t row_a = (uint64_t)lo[2] * J.nb[1] + lo[1];
-      uint64_t row_b = (uint64_t)hi[2] * J.nb[1] + hi[1];
-      const KeyT* kk = keys + J.pbase;
-      if (l == 0) {
-        const bool tab = J.cbase != kNoTable;
-        uint32_t a = tab ? cstart[J.cbase + row_a * J.nb[0]] : lower_bound_key(kk, n, row_a * J.nb[0]);
-        uint32_t b = tab ? cstart[J.cbase + (row_b + 1) * J.nb[0]] : lower_bound_key(kk, n, (row_b + 1) * J.nb[0]);
-        my_lo = a;
-        my_len = b - a;
-      }
-      n_ranges = 1;
-    }
-  }
-
-  // flattened candidate index space over the ranges: lane k < n_ranges
-  // holds range k as (first sorted position, first flattened index)
-  const uint32_t incl = wave_incl_scan(my_len);
-  const uint32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
-  const uint32_t my_ps = incl - my_len;
-
-  float sq[3][Q];
-#pragma unroll
-  for (int q = 0; q < Q; ++q)
-#pragma unroll
-    for (int e = 0; e < D; ++e) sq[e][q] = (RAW || exact) ? xq[e][q] : (xq[e][q] - o[e]) * hinv[e];
-
-  float S[Q], C[Q];
-#pragma unroll
-  for (int q = 0; q < Q; ++q) { S[q] = 0.0f; C[q] = 0.0f; }
-
-  // candidate loader for the chunk of flattened indices [cb, cb + 64): lane l
-  // loads index cb + l as raw coordinates (converted to local coordinates
-  // only when staged, so the load's latency overlaps the previous chunk's
-  // pair loop).  Range k (read from lane k) covers [ps_k, ps_k + len_k);
-  // the range of an index is the last one starting at or before it (empty
-  // ranges share their successor's start and are overridden by it).  Chunks
-  // come in increasing order, so the scan resumes at kc, the last range
-  // starting at or before the previous chunk, and stops at the first range
-  // starting past this chunk: a chunk visits the ranges that start inside it.
-  uint32_t kc = 0;
-  // range kc's offset and the next range's start, kept between chunks: a
-  // chunk that ends before the next range starts needs no scan
-  uint32_t kc_off = __builtin_amdgcn_readlane(my_lo, 0) - __builtin_amdgcn_readlane(my_ps, 0);
-  uint32_t next_ps = n_ranges > 1 ? __builtin_amdgcn_readlane(my_ps, 1) : 0xFFFFFFFFu;
-  auto load_cand = [&](uint32_t cb, float* x) {
-    const uint32_t f = cb + (uint32_t)l;
-    const uint32_t cend = cb + kWave;
-    const bool cend_wraps = cend < cb;
-    uint32_t off = 0;  // sorted position - flattened index, mod 2^32
-    if (!cend_wraps && next_ps >= cend) {
-      off = kc_off;
-    } else {
-      for (uint32_t k = kc; k < n_ranges; ++k) {
-        const uint32_t ps_k = __builtin_amdgcn_readlane(my_ps, k);
-        if (!cend_wraps && ps_k >= cend) break;
-        const uint32_t lo_k = __builtin_amdgcn_readlane(my_lo, k);
-        if (ps_k <= cb) kc = k;
-        if (f >= ps_k) off = lo_k - ps_k;
-      }
-      kc_off = __builtin_amdgcn_readlane(my_lo, kc) - __builtin_amdgcn_readlane(my_ps, kc);
-      next_ps = kc + 1 < n_ranges ? __builtin_amdgcn_readlane(my_ps, kc + 1) : 0xFFFFFFFFu;
-    }
-    if (f < total) {
-      const uint32_t p = f + off;
-      // 32-bit byte offsets from the dims' uniform base addresses (a job
-      // holds fewer than 2^30 points, so a coordinate column is below 4 GiB)
-#pragma unroll
-      for (int e = 0; e < D; ++e)
-        x[e] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(src[e]) + (uint32_t)(p * 4u));
-    } else {
-#pragma unroll
-      for (int e = 0; e < D; ++e) x[e] = kSentinel;
-    }
-  };
-
-  // one staged chunk: convert the loaded candidates, stage them, reload the
-  // buffer `ahead` chunks further on, run the pair loop
-  // exact_tag: std::true_type stages raw coordinates (u from raw differences)
-  auto chunk = [&](auto exact_tag, uint32_t base, float* buf, uint32_t ahead) {
-    constexpr bool kRawPairs = RAW || decltype(exact_tag)::value;
-    float s_c[3];
-#pragma unroll
-    for (int e = 0; e < D; ++e) s_c[e] = kRawPairs ? buf[e] : (buf[e] - o[e]) * hinv[e];
-    wave_sync();
-    stage_store<D>(L, l, s_c);
-    wave_sync();
-    if (base + ahead < total) load_cand(base + ahead, buf);
-    if constexpr (D == 1) {
-      if (tri) accumulate_chunk<1, Q, kRawPairs, true>(L.cand, sq, hinv, S, C);
-      else accumulate_chunk<1, Q, kRawPairs, false>(L.cand, sq, hinv, S, C);
-    } else {
-      accumulate_chunk<D, Q, kRawPairs>(L.cand, sq, hinv, S, C);
-    }
-  };
-  float nxt[3];
-  load_cand(0u, nxt);
-  if constexpr (D == 1 && !RAW) {
-    if (m_n != 0) {
-      moment_boxes<Q>(J, mom, m_lo, m_n, xq, S, C, L);
-      mstat[0] = __builtin_amdgcn_readlane(my_lo, 1) -
-                 (__builtin_amdgcn_readlane(my_lo, 0) + __builtin_amdgcn_readlane(my_len, 0));
-      mstat[1] = m_n;
-    }
-  }
-  if constexpr (D == 2 && !RAW) {
-    if (c_nr != 0) {
-      cell_moments<Q>(J, mom2, c_r0, c_nr, c_c0, c_nc, xq, S, C, L);
-      mstat[0] = c_pts;
-      mstat[1] = c_nr * c_nc;
-    }
-  }
-  if (!RAW && exact) mstat[2] = 1u;
-  if (!RAW && exact)
-    for (uint32_t base = 0; base < total; base += kWave) chunk(std::true_type{}, base, nxt, kWave);
-  else
-    for (uint32_t base = 0; base < total; base += kWave) chunk(std::false_type{}, base, nxt, kWave);
-  wave_sync();
-
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    if (qv[q]) dens[J.out_off + slot[q]] = ((double)S[q] - (double)C[q]) * J.norm;
-  }
-  return total;
-}
-
-// Densities of a packed tile: G sub-tiles of at most W = 64 / G queries,
-// one query per lane, lanes [s W, s W + W) on sub-tile s.  Each sub-tile has
-// its own bounding box, candidate ranges (held in the wave's LDS range
-// table) and stage slice of W candidates; the wave steps through the
-// longest sub-tile's candidates W at a time.  Returns the executed pairs.
-template <int D, int G, typename KeyT>
-__device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __restrict__ scoord,
-                                 const KeyT* __restrict__ keys, const uint32_t* __restrict__ cstart,
-                                 const uint32_t* __restrict__ idx, double* __restrict__ dens, WaveLds& L,
-                                 uint64_t& lane_slots, uint64_t& raw_tiles) {
-  constexpr int W = kWave / G;
-  constexpr int kPer4 = D == 2 ? 2 : 1;
-  const int l = lane_id();
-  const int s = l / W, j = l % W;
-  const uint32_t n = J.n;
-  const float* src[3];
-#pragma unroll
-  for (int e = 0; e < D; ++e) src[e] = scoord + 3 * J.pbase + (uint64_t)e * n;
-  float hinv[3] = {J.hinv[0], J.hinv[1], J.hinv[2]};
-
-  // this lane's sub-tile and query
-  const uint32_t span = T.p1 - T.p0;
-  const uint32_t b0 = s == 0 ? 0u : (T.pack >> (8 * s)) & 0xFFu;
-  const uint32_t b1 = s + 1 == G ? span : (T.pack >> (8 * (s + 1))) & 0xFFu;
-  const uint32_t nq = b1 - b0;
-  const bool qv = (uint32_t)j < nq;
-  const uint32_t p = T.p0 + b0 + (qv ? (uint32_t)j : 0u);
-  float xq[3];
-#pragma unroll
-  for (int e = 0; e < D; ++e) xq[e] = src[e][p];
-  const uint32_t slot = idx[J.pbase + p];
-
-  // sub-tile bounding box (idle lanes repeat the sub-tile's first query);
-  // its centre is the sub-tile's origin.  A pack with a sub-tile wider than
-  // kLocalSpan bandwidths along dim 0 uses raw-coordinate pair terms.
-  float o[3] = {0.f, 0.f, 0.f};
-  uint32_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-  float span0 = 0.f;
-#pragma unroll
-  for (int e = 0; e < D; ++e) {
-    const float mn = sub_min<W>(xq[e]);
-    const float mx = sub_max<W>(xq[e]);
-    box_range(mn, mx, J.h[e], J.mn[e], J.bs[e], J.nb[e], lo[e], hi[e]);
-    o[e] = 0.5f * (mn + mx);
-    if (e == 0) span0 = (mx - mn) * J.hinv[0];
-  }
-  const bool exact = wave_max(span0) > kLocalSpan;
-  const uint32_t n1 = hi[1] - lo[1] + 1;
-  const uint32_t n2 = (D == 3) ? (hi[2] - lo[2] + 1) : 1u;
-  uint32_t nr = n1 * n2;
-  uint32_t my_lo = 0, my_len = 0;
-  const bool tab = J.cbase != kNoTable;
-  const KeyT* kk = keys + J.pbase;
-  if (nr <= (uint32_t)W) {
-    if ((uint32_t)j < nr) {
-      const uint32_t r1 = lo[1] + (uint32_t)j % n1;
-      const uint32_t r2 = (D == 3) ? lo[2] + (uint32_t)j / n1 : 0u;
-      const uint64_t row = (uint64_t)r2 * J.nb[1] + r1;
-      const uint64_t klo = row * J.nb[0] + lo[0];
-      const uint64_t khi = row * J.nb[0] + hi[0];
-      uint32_t a, b;
-      if (tab) {
-        a = cstart[J.cbase + klo];
-        b = cstart[J.cbase + khi + 1];
-      } else {
-        a = lower_bound_key(kk, n, klo);
-        b = a + lower_bound_key(kk + a, n - a, khi + 1);
-      }
-      my_lo = a;
-      my_len = b - a;
-    }
-  } else {
-    // more candidate rows than lanes: one range over every row between the
-    // first and last candidate row (a superset)
-    if (j == 0) {
-      const uint64_t row_a = (uint64_t)lo[2] * J.nb[1] + lo[1];
-      const uint64_t row_b = (uint64_t)hi[2] * J.nb[1] + hi[1];
-      const uint32_t a = tab ? cstart[J.cbase + row_a * J.nb[0]] : lower_bound_key(kk, n, row_a * J.nb[0]);
-      const uint32_t b =
-          tab ? cstart[J.cbase + (row_b + 1) * J.nb[0]] : lower_bound_key(kk, n, (row_b + 1) * J.nb[0]);
-      my_lo = a;
-      my_len = b - a;
-    }
-    nr = 1;
-  }
-  // flattened candidate index space of the sub-tile; the range table goes
-  // to LDS, entry s W + k = (first flattened index, first position) of range k
-  const uint32_t incl = sub_incl_scan<W>(my_len);
-  const uint32_t total = sub_sum<W>(my_len);
-  wave_sync();
-  L.rng[l] = make_uint2(incl - my_len, my_lo);
-  wave_sync();
-  const uint32_t steps = wave_max_u32((total + W - 1) / W);
-
-  float sq[3][1];
-#pragma unroll
-  for (int e = 0; e < D; ++e) sq[e][0] = exact ? xq[e] : (xq[e] - o[e]) * hinv[e];
-  float S[1] = {0.0f}, C[1] = {0.0f};
-
-  // candidate f of the sub-tile: range k is the last one starting at or
-  // before f; steps come in increasing order, so kc only moves forward
-  uint32_t kc = 0;
-  const uint2* rng = L.rng + s * W;
-  auto load_cand = [&](uint32_t step, float* x) {
-    const uint32_t f = step * W + (uint32_t)j;
-    if (f < total) {
-      while (kc + 1 < nr && rng[kc + 1].x <= f) ++kc;
-      const uint2 r = rng[kc];
-      const uint32_t pos = f - r.x + r.y;
-#pragma unroll
-      for (int e = 0; e < D; ++e) x[e] = src[e][pos];
-    } else {
-#pragma unroll
-      for (int e = 0; e < D; ++e) x[e] = kSentinel;
-    }
-  };
-  const float4* stage = L.cand + s * (W / kPer4);
-  float nxt[3];
-  load_cand(0u, nxt);
-  auto run = [&](auto exact_tag) {
-    constexpr bool kRawPairs = decltype(exact_tag)::value;
-    for (uint32_t b = 0; b < steps; ++b) {
-      float s_c[3];
-#pragma unroll
-      for (int e = 0; e < D; ++e) s_c[e] = kRawPairs ? nxt[e] : (nxt[e] - o[e]) * hinv[e];
-      wave_sync();
-      stage_store<D>(L, l, s_c);
-      wave_sync();
-      if (b + 1 < steps) load_cand(b + 1, nxt);
-      accumulate_chunk<D, 1, kRawPairs, false, W>(stage, sq, hinv, S, C);
-    }
-  };
-  if (exact) {
-    ++raw_tiles;
-    run(std::true_type{});
-  } else {
-    run(std::false_type{});
-  }
-  wave_sync();
-
-  if (qv) dens[J.out_off + slot] = ((double)S[0] - (double)C[0]) * J.norm;
-  lane_slots += (uint64_t)kWave * W * steps;
-  uint64_t pr = 0;
-#pragma unroll
-  for (int g = 0; g < G; ++g)
-    pr += (uint64_t)__builtin_amdgcn_readlane(nq, g * W) * __builtin_amdgcn_readlane(total, g * W);
-  return pr;
-}
-
-// Queries per lane chosen per tile: the smallest Q <= QMAX whose 64*Q lane
-// slots hold the tile (Q = 4 steps down through 3, 2 and 1), so partly
-// filled tiles do not pay for idle slots.
-// pairs counts every candidate pair of the tile; macc[d-1] those summed from
-// moments in d-dim jobs (d = 1, 2), macc[d+1] the (query, box) / (query,
-// cell) moment terms evaluated for them, macc[4] the tiles with raw-coordinate
-// pair terms.
-template <int D, int QMAX, bool RAW, typename KeyT>
-__device__ __forceinline__ void run_tile(const DJob& J, const DTile& T, const float* __restrict__ X,
-                                         const float* __restrict__ scoord, const KeyT* __restrict__ keys,
-                                         const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ idx,
-                                         const float4* __restrict__ mom, const float* __restrict__ mom2,
-                                         double* __restrict__ dens, WaveLds& L, uint64_t& pairs,
-                                         uint64_t& lane_slots, uint64_t (&macc)[5]) {
-  const uint32_t nq = T.p1 - T.p0;
-  if constexpr (QMAX >= 2 && !RAW) {  // plain-path tiles are full except the last
-    if (nq <= (uint32_t)(kWave * (QMAX - 1))) {
-      run_tile<D, QMAX - 1, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, mom2, dens, L, pairs, lane_slots,
-                                       macc);
-      return;
-    }
-  }
-  uint32_t mstat[3] = {0u, 0u, 0u};
-  const uint32_t total =
-      process_tile<D, QMAX, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, mom2, dens, L, mstat);
-  pairs += (uint64_t)nq * (total + mstat[0]);
-  lane_slots += (uint64_t)(kWave * QMAX) * ((total + kWave - 1) / kWave * kWave);
-  if constexpr (D <= 2 && !RAW) {
-    macc[D - 1] += (uint64_t)nq * mstat[0];
-    macc[D + 1] += (uint64_t)nq * mstat[1];
-  }
-  macc[4] += mstat[2];
-}
-
-}  // namespace
-
-// Persistent density kernel: every wave pulls tiles from the claim counters
-// until the tile list is exhausted.
-template <int Q, bool RAW, typename KeyT>
-__global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const DJob* __restrict__ jobs,
-                                                    const DTile* __restrict__ tiles,
-                                                    const uint32_t* __restrict__ n_tiles_p,
-                                                    uint32_t* __restrict__ counter,
-                                                    const float* __restrict__ X,
-                                                    const float* __restrict__ scoord,
-                                                    const KeyT* __restrict__ keys,
-                                                    const uint32_t* __restrict__ cstart,
-                                                    const uint32_t* __restrict__ idx,
-                                                    const float4* __restrict__ mom,
-                                                    const float* __restrict__ mom2,
-                                                    double* __restrict__ dens,
-                                                    unsigned long long* __restrict__ pairs) {
-  __shared__ WaveLds lds[kBlock / kWave];
-  WaveLds& L = lds[threadIdx.x / kWave];
-  const uint32_t n_tiles = *n_tiles_p;
-  const int l = lane_id();
-  // candidate pairs per dimension, lane slots and the moment counts, flushed
-  // once per wave
-  uint64_t acc[3] = {0, 0, 0}, slots = 0, macc[5] = {0, 0, 0, 0, 0};
-  // Tiles are claimed from kClaimParts contiguous partitions of the tile
-  // list, each with its own counter: a wave starts on the partition of its
-  // workgroup's XCD (workgroups are dealt to the XCDs round robin) and moves
-  // on to the others once that one is exhausted.  One shared counter
-  // serialises the claims of short tiles at the memory's atomic unit.
-  const uint32_t home = blockIdx.x % kClaimParts;
-  for (uint32_t k = 0; k < kClaimParts; ++k) {
-    const uint32_t part = (home + k) % kClaimParts;
-    const uint32_t lo = (uint32_t)((uint64_t)n_tiles * part / kClaimParts);
-    const uint32_t hi = (uint32_t)((uint64_t)n_tiles * (part + 1) / kClaimParts);
-    uint32_t* ctr = counter + part * kClaimStride;
-    for (;;) {
-      uint32_t i = 0;
-      if (l == 0) i = atomicAdd(ctr, 1u);
-      i = __builtin_amdgcn_readfirstlane(i);
-      if (i >= hi - lo) break;
-      const uint32_t t = lo + i;
-      const DTile T = tiles[t];
-      const uint32_t pk = __builtin_amdgcn_readfirstlane(T.pack);
-      if (pk == kTileSkip) continue;
-      const DJob J = jobs[T.job];
-      const uint32_t d = __builtin_amdgcn_readfirstlane(J.d);
-      if constexpr (!RAW) {
-        if (pk & 3u) {
-          const bool four = (pk & 3u) == 3u;
-          if (d == 2)
-            acc[1] += four ? process_pack<2, 4, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots, macc[4])
-                           : process_pack<2, 2, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots, macc[4]);
-          else
-            acc[2] += four ? process_pack<3, 4, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots, macc[4])
-                           : process_pack<3, 2, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, slots, macc[4]);
-          continue;
-        }
-      }
-      if (d == 1)
-        run_tile<1, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, mom2, dens, L, acc[0], slots, macc);
-      else if (d == 2)
-        run_tile<2, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, mom2, dens, L, acc[1], slots, macc);
-      else
-        run_tile<3, Q, RAW, KeyT>(J, T, X, scoord, keys, cstart, idx, mom, mom2, dens, L, acc[2], slots, macc);
-    }
-  }
-  if (l == 0) {
-    for (int e = 0; e < 3; ++e)
-      if (acc[e]) atomicAdd(pairs + e, (unsigned long long)acc[e]);
-    if (slots) atomicAdd(pairs + 3, (unsigned long long)slots);
-    for (int e = 0; e < 5; ++e)
-      if (macc[e]) atomicAdd(pairs + 4 + e, (unsigned long long)macc[e]);
   }
 }
 
@@ -1949,13 +1006,13 @@ void launch_gather_sorted(hipStream_t s, const DJob* jobs, const uint2* chunks, 
 }
 
 void launch_tile_write(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const void* keys, uint32_t tq, const uint32_t* offs, DTile* tiles) {
+                       const void* keys, const uint32_t* offs, DTile* tiles) {
   if (wide)
     hipLaunchKernelGGL(k_tile_write<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint64_t*)keys, tq, offs, tiles);
+                       n_chunks, (const uint64_t*)keys, offs, tiles);
   else
     hipLaunchKernelGGL(k_tile_write<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint32_t*)keys, tq, offs, tiles);
+                       n_chunks, (const uint32_t*)keys, offs, tiles);
 }
 
 void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
@@ -1973,42 +1030,15 @@ void launch_tile_pack(hipStream_t s, const DJob* jobs, DTile* tiles, const uint3
   hipLaunchKernelGGL(k_tile_pack, dim3((unsigned)blocks), dim3(kBlock), 0, s, jobs, tiles, total);
 }
 
-// Kernel variants: RAW (plain path, keys unused) or cell lists with 32- or
-// 64-bit keys, for q = 1, 2, 4, 8 queries per lane.
-template <int Q>
-static const void* density_fn(bool raw, bool wide) {
-  if (raw) return reinterpret_cast<const void*>(&k_density<Q, true, uint32_t>);
-  if (wide) return reinterpret_cast<const void*>(&k_density<Q, false, uint64_t>);
-  return reinterpret_cast<const void*>(&k_density<Q, false, uint32_t>);
-}
-
+// Density kernel variants for q = 1, 2, 4 queries per lane, each compiled
+// in its own file (kde_density_q*.hip).
 int density_occupancy(int q, bool raw, bool wide, int* blocks_per_cu) {
-  const void* fn = nullptr;
   switch (q) {
-    case 1: fn = density_fn<1>(raw, wide); break;
-    case 2: fn = density_fn<2>(raw, wide); break;
-    case 4: fn = density_fn<4>(raw, wide); break;
+    case 1: return density_occupancy_q<1>(raw, wide, blocks_per_cu);
+    case 2: return density_occupancy_q<2>(raw, wide, blocks_per_cu);
+    case 4: return density_occupancy_q<4>(raw, wide, blocks_per_cu);
     default: return -1;
   }
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, kBlock, 0);
-  return e == hipSuccess ? 0 : -1;
-}
-
-template <int Q>
-static void launch_density_q(hipStream_t s, bool raw, bool wide, unsigned grid, const DJob* jobs,
-                             const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
-                             const float* scoord, const void* keys, const uint32_t* cstart,
-                             const uint32_t* idx, const float4* mom, const float* mom2, double* dens,
-                             unsigned long long* pairs) {
-  if (raw)
-    hipLaunchKernelGGL((k_density<Q, true, uint32_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
-                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, mom, mom2, dens, pairs);
-  else if (wide)
-    hipLaunchKernelGGL((k_density<Q, false, uint64_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
-                       counter, X, scoord, (const uint64_t*)keys, cstart, idx, mom, mom2, dens, pairs);
-  else
-    hipLaunchKernelGGL((k_density<Q, false, uint32_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
-                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, mom, mom2, dens, pairs);
 }
 
 int launch_density(hipStream_t s, int q, bool raw, bool wide, unsigned grid, const DJob* jobs,
@@ -2045,13 +1075,13 @@ void launch_cell_moments2(hipStream_t s, const DJob* jobs, const uint32_t* mlist
 }
 
 void launch_cell_starts(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                        const void* keys, uint32_t tq, uint32_t* tile_cnt, uint32_t* start) {
+                        const void* keys, uint32_t* tile_cnt, uint32_t* start) {
   if (wide)
     hipLaunchKernelGGL(k_cell_starts<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint64_t*)keys, tq, tile_cnt, start);
+                       n_chunks, (const uint64_t*)keys, tile_cnt, start);
   else
     hipLaunchKernelGGL(k_cell_starts<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint32_t*)keys, tq, tile_cnt, start);
+                       n_chunks, (const uint32_t*)keys, tile_cnt, start);
 }
 
 }  // namespace ent

@@ -34,10 +34,10 @@ void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uin
                            uint32_t* scratch);
 void launch_gather_sorted(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
                           const float* X, const uint32_t* idx, float* scoord);
-// Tile boundaries: every tq points, at row changes and between dim-0 groups
-// (counted per chunk by launch_cell_starts).
+// Tile boundaries: every job.tq points, at row changes and between dim-0
+// groups (counted per chunk by launch_cell_starts).
 void launch_tile_write(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const void* keys, uint32_t tq, const uint32_t* offs, DTile* tiles);
+                       const void* keys, const uint32_t* offs, DTile* tiles);
 void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
                       uint64_t max_tiles);
 // Packs runs of small tiles of one job for side-by-side processing (in place).
@@ -56,6 +56,14 @@ constexpr uint32_t kClaimStride = 1024;
 constexpr uint32_t kCounterWords = kClaimPartsMax * kClaimStride;
 static_assert(kClaimParts >= 1 && kClaimParts <= kClaimPartsMax, "claim partitions");
 int density_occupancy(int q, bool raw, bool wide, int* blocks_per_cu);
+// One queries-per-lane setting (instantiated in kde_density_q<Q>.hip).
+template <int Q>
+int density_occupancy_q(bool raw, bool wide, int* blocks_per_cu);
+template <int Q>
+void launch_density_q(hipStream_t s, bool raw, bool wide, unsigned grid, const DJob* jobs, const DTile* tiles,
+                      const uint32_t* n_tiles, uint32_t* counter, const float* X, const float* scoord,
+                      const void* keys, const uint32_t* cstart, const uint32_t* idx, const float4* mom,
+                      const float* mom2, double* dens, unsigned long long* pairs);
 int launch_density(hipStream_t s, int q, bool raw, bool wide, unsigned grid, const DJob* jobs,
                    const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
                    const float* scoord, const void* keys, const uint32_t* cstart, const uint32_t* idx,
@@ -74,6 +82,6 @@ void launch_cell_moments2(hipStream_t s, const DJob* jobs, const uint32_t* mlist
 // Cell-start tables of the jobs that have one, and the tile count of every
 // chunk (chunks in tile order) into tile_cnt, in one pass over the keys.
 void launch_cell_starts(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                        const void* keys, uint32_t tq, uint32_t* tile_cnt, uint32_t* start);
+                        const void* keys, uint32_t* tile_cnt, uint32_t* start);
 
 }  // namespace ent

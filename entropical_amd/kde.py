@@ -65,11 +65,12 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
-    def set_tuning(self, q: int = 2, m0: int = 8, mr: int = 1, group_h: int = 4) -> None:
-        check(load().ent_ctx_set_tuning(self.handle, q, m0, mr, group_h))
+    def set_tuning(self, q: int = 2, m0: int = 8, mr: int = 1, group_h: int = 4, sub: int = 0) -> None:
+        check(load().ent_ctx_set_tuning(self.handle, q, m0, mr, group_h, sub))
 
     def get_tuning(self) -> tuple:
-        t = (C.c_int * 4)()
+        """(q, m0, mr, group_h, sub)."""
+        t = (C.c_int * 5)()
         check(load().ent_ctx_get_tuning(self.handle, t))
         return tuple(int(v) for v in t)
 
@@ -404,7 +405,7 @@ def mi_autotune(ctx: Context, x: DeviceArray, h: Sequence[float]) -> tuple:
     """Tune the engine for the MI job set of ``x`` (kept on ``ctx``)."""
     n_cols, n_rows = x.shape
     hh = _f32(h)
-    t = (C.c_int * 4)()
+    t = (C.c_int * 5)()
     check(load().ent_mi_autotune_dev(ctx.handle, C.c_void_p(x.ptr), n_rows, n_cols, _ptr(hh), t))
     return tuple(int(v) for v in t)
 
@@ -413,7 +414,7 @@ def te_autotune(ctx: Context, x: DeviceArray, h: Sequence[float], taus: Sequence
     """Tune the engine for the transfer-entropy job set (kept on ``ctx``)."""
     n_cols, n_rows = x.shape
     hh, tt = _f32(h), _u32(taus)
-    t = (C.c_int * 4)()
+    t = (C.c_int * 5)()
     check(load().ent_te_autotune_dev(ctx.handle, C.c_void_p(x.ptr), n_rows, n_cols, _ptr(hh), _ptr(tt),
                                      tt.size, t))
     return tuple(int(v) for v in t)
@@ -422,11 +423,11 @@ def te_autotune(ctx: Context, x: DeviceArray, h: Sequence[float], taus: Sequence
 def batch_autotune(ctx: Context, x_dev_ptr: int, specs: Sequence[dict], return_ms: bool = False):
     """Tune the engine for an arbitrary batch of jobs (kept on ``ctx``).
 
-    Returns the tuning (q, m0, mr, group_h); with ``return_ms`` also the
+    Returns the tuning (q, m0, mr, group_h, sub); with ``return_ms`` also the
     winner's time on the whole batch in ms (-1 when the tuning came from the
     cache of an earlier search of the same workload signature)."""
     arr = make_jobs(specs)
-    t = (C.c_int * 4)()
+    t = (C.c_int * 5)()
     ms = C.c_double()
     check(load().ent_autotune_dev(ctx.handle, C.c_void_p(x_dev_ptr), arr, len(specs), t, C.byref(ms)))
     tun = tuple(int(v) for v in t)

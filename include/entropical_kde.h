@@ -72,9 +72,11 @@ int ent_ctx_create(int device, ent_ctx** out);
 int ent_ctx_destroy(ent_ctx* ctx);
 
 /* Engine tuning: maximum queries per lane (1,2,4; smaller tiles use fewer), dim-0 boxes per bandwidth,
- * row-dim boxes per bandwidth, tile extent bound in bandwidths.  Replaces
- * the reference's compile-time BSIZE (densities_cuda.cu:14). */
-int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h);
+ * row-dim boxes per bandwidth, tile extent bound in bandwidths, and the
+ * sub-tile size of 2-D jobs (0: whole tiles; 16 or 32: tiles of up to 64
+ * queries run as sub-tiles of that many queries, each with its own candidate
+ * stream).  Replaces the reference's compile-time BSIZE (densities_cuda.cu:14). */
+int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h, int sub);
 
 /* Statistics of the last density run: persistent-kernel time and whole
  * pipeline time (HIP events, ms), candidate pair evaluations (the pairs of
@@ -273,22 +275,25 @@ int ent_hestimate_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_
 
 /* Tile-shape autotuner (no reference counterpart; the reference fixes its
  * tile at compile time, densities_cuda.cu:14).  Times every setting of the
- * grid q in {1,2,4}, m0 in {8,16,32}, mr in {1,2}, group_h in {4,16,64} on a
+ * grid q in {1,2,4}, m0 in {8,16,32}, mr in {1,2}, group_h in {4,16,64} with
+ * whole 2-D tiles, and for batches with 2-D jobs also q in {1,2}, m0 in
+ * {16,32}, mr in {2,4}, group_h = 64 with 2-D sub-tiles of 16, on a
  * subsample of the job set (the four fastest re-timed on the whole set),
- * keeps the fastest on the context (and returns it in tuning4 when non-NULL),
+ * keeps the fastest on the context (and returns it in tuning5 = (q, m0, mr,
+ * group_h, sub) when non-NULL),
  * and caches it per workload signature (file named by ENT_TUNE_CACHE, if set).
  * The MI/TE variants tune on the job sets of ent_mi_matrix_dev /
  * ent_te_matrix_dev; the generic one on an arbitrary batch (best_ms: time of
  * the winning setting on the subsample, -1 on a cache hit). */
 int ent_mi_autotune_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
-                        int* tuning4);
+                        int* tuning5);
 int ent_te_autotune_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_cols, const float* h,
-                        const uint32_t* taus, size_t n_taus, int* tuning4);
-int ent_autotune_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs, int* tuning4,
+                        const uint32_t* taus, size_t n_taus, int* tuning5);
+int ent_autotune_dev(ent_ctx* ctx, const float* X_dev, const ent_job* jobs, size_t n_jobs, int* tuning5,
                      double* best_ms);
 
-/* Current engine tuning (q, m0, mr, group_h). */
-int ent_ctx_get_tuning(ent_ctx* ctx, int* tuning4);
+/* Current engine tuning (q, m0, mr, group_h, sub) into tuning5. */
+int ent_ctx_get_tuning(ent_ctx* ctx, int* tuning5);
 
 /* Wall time (ms) of the last driver call on this context. */
 int ent_ctx_last_driver_ms(ent_ctx* ctx, double* ms);

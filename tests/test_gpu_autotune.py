@@ -14,11 +14,15 @@ from entropical_amd import (Context, DeviceArray, EntropicalError, batch_autotun
 pytestmark = pytest.mark.gpu
 
 GRID_Q, GRID_M0, GRID_MR, GRID_GH = (1, 2, 4), (8, 16, 32), (1, 2), (4, 16, 64)
+# 2-D sub-tiles of 16 (batches with 2-D jobs)
+SUB_Q, SUB_M0, SUB_MR = (1, 2), (16, 32), (2, 3, 4)
 
 
-def in_grid(t):
-    q, m0, mr, gh = t
-    return q in GRID_Q and m0 in GRID_M0 and mr in GRID_MR and gh in GRID_GH
+def in_grid(t, has_2d=True):
+    q, m0, mr, gh, sub = t
+    if sub == 0:
+        return q in GRID_Q and m0 in GRID_M0 and mr in GRID_MR and gh in GRID_GH
+    return has_2d and sub == 16 and q in SUB_Q and m0 in SUB_M0 and mr in SUB_MR and gh == 64
 
 
 def rel_err(a, b):
@@ -103,14 +107,14 @@ def test_autotune_skips_unsupported_shapes_and_restores_on_failure(ctx, orc):
     try:
         ctx.set_tuning(2, 8, 1, 4)
         t = batch_autotune(ctx, dx.ptr, [{"n_dim": 1, "n": n, "off": [0], "h": [1.0]}])
-        assert in_grid(t) and t[1] in (8, 16), t
+        assert in_grid(t, has_2d=False) and t[1] in (8, 16) and t[4] == 0, t
         got = combined_densities(x, n, [0], [1.0], ctx=ctx)
         assert rel_err(got, orc.density_fp64(x, n, [0], [1.0])) < 1e-5
         # range / h = 1e7: no grid point fits; the error leaves the caller's tuning
-        ctx.set_tuning(4, 16, 2, 16)
+        ctx.set_tuning(4, 16, 2, 16, 16)
         with pytest.raises(EntropicalError):
             batch_autotune(ctx, dx.ptr, [{"n_dim": 1, "n": n, "off": [0], "h": [0.1]}])
-        assert ctx.get_tuning() == (4, 16, 2, 16)
+        assert ctx.get_tuning() == (4, 16, 2, 16, 16)
     finally:
         dx.free()
         ctx.set_tuning()

@@ -92,7 +92,8 @@ def test_all_points_identical(ctx):
     np.testing.assert_allclose(got, (0.75 / h1) * (0.75 / h2), rtol=1e-6)
 
 
-@pytest.mark.parametrize("tuning", [(1, 1, 1, 1), (2, 2, 1, 4), (4, 8, 2, 1), (4, 4, 3, 2)])
+@pytest.mark.parametrize("tuning", [(1, 1, 1, 1), (2, 2, 1, 4), (4, 8, 2, 1), (4, 4, 3, 2), (1, 32, 4, 64, 16),
+                                    (2, 16, 2, 16, 32), (4, 8, 5, 2, 16)])
 def test_tunings_agree(ctx, orc, tuning):
     from entropical_amd import combined_densities
 
@@ -200,8 +201,10 @@ def test_cell_sort_bit_exact(ctx, orc, monkeypatch, sort_mode, scatter):
 def test_packed_small_tiles(ctx, orc, monkeypatch, d):
     """Sparse tails give tiles of a few queries, which run packed two or four
     to a wave (16 or 32 lanes per tile).  Densities with packing and without
-    it (ENT_TILE_PACK=0) both match FP64, and differ from each other in the
-    last bits (each sub-tile has its own origin), so the packed path ran."""
+    it (ENT_TILE_PACK=0) both match FP64, and packing issues fewer lane
+    slots, so the packed path ran.  A packed 2-D tile sums its sub-tiles'
+    candidates in the order of a whole tile (64 per step, a compensated
+    merge every 32, the same origin): bit-identical densities."""
     from entropical_amd import combined_densities
 
     r = np.random.default_rng(31 + d)
@@ -212,11 +215,15 @@ def test_packed_small_tiles(ctx, orc, monkeypatch, d):
     ref = orc.density_fp64(x, n, cols, h)
     monkeypatch.setenv("ENT_TILE_PACK", "0")
     single = combined_densities(x, n, cols, h, ctx=ctx)
+    slots_single = ctx.last_stats()["lane_pairs"]
     monkeypatch.setenv("ENT_TILE_PACK", "1")
     packed = combined_densities(x, n, cols, h, ctx=ctx)
+    slots_packed = ctx.last_stats()["lane_pairs"]
     assert rel_err(single, ref) <= RTOL
     assert rel_err(packed, ref) <= RTOL
-    assert not np.array_equal(single, packed)
+    assert slots_packed < slots_single
+    if d == 2:
+        np.testing.assert_array_equal(single, packed)
 
 
 @pytest.mark.gpu
@@ -274,6 +281,70 @@ def test_cell_moments_2d(ctx, orc, monkeypatch, h):
     assert st1["pairs"] == st0["pairs"]
     assert 0 < st1["moment_pairs_by_d"][1] < st1["pairs"]
     assert 0 < st1["moment_terms_by_d"][1] < st1["moment_pairs_by_d"][1]
+
+
+@pytest.mark.parametrize("sub,mr", [(16, 4), (32, 4), (16, 2)])
+def test_sub_tiles_2d(ctx, orc, monkeypatch, sub, mr):
+    """2-D tiles run as sub-tiles of 16 or 32 queries, each with its own
+    bounding box, moment cells and candidate stream.  With moments on and
+    off the densities match FP64 and equal the whole-tile path's within the
+    tolerance; the sub-tiles' candidate windows are smaller, so fewer pairs
+    are evaluated one by one than with whole tiles of the same shape; with
+    moments on, some pairs come from cell moments."""
+    from entropical_amd import combined_densities
+
+    r = np.random.default_rng(151 + sub + mr)
+    n = 80000
+    a = r.standard_normal(n)
+    x = np.stack([a, 0.6 * a + 0.8 * r.standard_normal(n)]).astype(np.float32)
+    h = [0.12, 0.15]
+    ref = orc.density_fp64_window(x, n, [0, 1], h)
+    got, st = {}, {}
+    try:
+        for s in (0, sub):
+            for mom in ("0", "1"):
+                monkeypatch.setenv("ENT_BOX_MOMENTS", mom)
+                ctx.set_tuning(1, 32, mr, 64, s)
+                got[s, mom] = combined_densities(x, n, [0, 1], h, ctx=ctx)
+                st[s, mom] = ctx.last_stats()
+                assert rel_err(got[s, mom], ref) < RTOL, (s, mom)
+    finally:
+        ctx.set_tuning()
+
+    def evaluated(t):
+        return t["pairs"] - t["moment_pairs"]
+
+    # same candidate windows with moments on and off
+    assert st[sub, "0"]["pairs"] == st[sub, "1"]["pairs"]
+    assert st[sub, "0"]["moment_pairs"] == 0
+    assert 0 < st[sub, "1"]["moment_pairs_by_d"][1] < st[sub, "1"]["pairs"]
+    assert evaluated(st[sub, "0"]) < 0.9 * evaluated(st[0, "0"])
+    assert evaluated(st[sub, "1"]) < 0.9 * evaluated(st[0, "1"])
+
+
+def test_sub_tiles_packed_and_sparse(ctx, orc, monkeypatch):
+    """Heavy tails give short 2-D tiles: those of at most 16 or 32 queries
+    are packed (ENT_TILE_PACK) and run through the sub-tile path with the
+    offsets of the pack, the others are split; tiles wider than 32
+    bandwidths take raw-coordinate pair terms.  All match FP64 with packing
+    on and off."""
+    from entropical_amd import combined_densities
+
+    r = np.random.default_rng(157)
+    n = 30000
+    x = (r.standard_t(1.5, size=(2, n)) * 2.0).astype(np.float32)
+    h = [0.05, 0.3]
+    ref = orc.density_fp64_window(x, n, [0, 1], h)
+    try:
+        for sub in (16, 32):
+            ctx.set_tuning(1, 32, 4, 64, sub)
+            for pack in ("0", "1"):
+                monkeypatch.setenv("ENT_TILE_PACK", pack)
+                got = combined_densities(x, n, [0, 1], h, ctx=ctx)
+                assert rel_err(got, ref) < RTOL, (sub, pack)
+            assert ctx.last_stats()["raw_tiles"] > 0
+    finally:
+        ctx.set_tuning()
 
 
 def test_moment_tables_over_cap_fall_back(ctx, orc, monkeypatch):

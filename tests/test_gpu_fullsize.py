@@ -44,10 +44,12 @@ def test_mi_matrix_64x200000(ctx, orc, threads):
         assert abs(mi[i, j] - ref) <= RTOL * abs(ref), (i, j, mi[i, j], ref)
 
 
-def test_mi_densities_64x200000(ctx, orc, threads):
-    """The MI batch at 64 x 200k with the benchmark's tile shape (cell and box
-    moments active): per-point densities of three 2-D jobs and two 1-D jobs
-    within 1e-5 of FP64 (mi.cpp:37-52 calls; densities_cuda.cu:46-105)."""
+@pytest.mark.parametrize("tuning", [(1, 32, 2, 64, 0), (1, 32, 3, 64, 16)])
+def test_mi_densities_64x200000(ctx, orc, threads, tuning):
+    """The MI batch at 64 x 200k with whole tiles and with 2-D sub-tiles of 16
+    (cell and box moments active): per-point densities of three 2-D jobs and
+    two 1-D jobs within 1e-5 of FP64 (mi.cpp:37-52 calls;
+    densities_cuda.cu:46-105)."""
     from _jobsets import mi_jobs
 
     from entropical_amd import DeviceArray, batch_densities_dev, datasets
@@ -61,7 +63,7 @@ def test_mi_densities_64x200000(ctx, orc, threads):
     keys = ((0, 1), (17, 42), (5, 63), (0,), (42,))
     got = {}
     try:
-        ctx.set_tuning(1, 32, 2, 64)
+        ctx.set_tuning(*tuning)
         batch_densities_dev(ctx, dx.ptr, jobs, out)
         st = ctx.last_stats()
         for k in keys:

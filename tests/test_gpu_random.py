@@ -8,7 +8,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 RTOL = 1e-5
-TUNINGS = [(1, 32, 2, 64), (2, 32, 1, 64), (1, 16, 2, 16), (4, 8, 1, 4), (2, 64, 2, 8), (1, 12, 3, 5)]
+TUNINGS = [(1, 32, 2, 64), (2, 32, 1, 64), (1, 16, 2, 16), (4, 8, 1, 4), (2, 64, 2, 8), (1, 12, 3, 5),
+           (1, 32, 4, 64, 16), (2, 16, 4, 16, 32)]
 
 
 def sample(r, n):
@@ -37,7 +38,7 @@ def test_random_jobs(ctx, orc, monkeypatch):
     r = np.random.default_rng(20261203)
     failures = []
     try:
-        for case in range(48):
+        for case in range(64):
             d = int(r.integers(1, 4))
             n = int(r.choice([300, 3000, 20000]))
             x, kind = sample(r, n)
