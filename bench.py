@@ -217,6 +217,21 @@ def rates(st, sec: float) -> dict:
             "pairs_evaluated_by_dim": evaluated_pairs(st), "candidate_gpair_per_s": cand / sec / 1e9}
 
 
+def support_fields(r: dict, in_support: int, sec: float) -> dict:
+    """Pairs inside the kernel's support (ordered (query, point) pairs with a
+    nonzero term, the self pair included; exact count on the host): the work
+    every exact method accounts for, whatever it evaluates.  The pairs summed
+    from moments all lie inside it, so the in-support share of the pairs
+    evaluated one by one is (in support - by moments) / evaluated."""
+    return {
+        "pairs_in_support": in_support,
+        "in_support_gpair_per_s": in_support / sec / 1e9,
+        "in_support_of_candidates": in_support / r["pairs_candidate"],
+        "in_support_of_evaluated": (in_support - r["pairs_by_moments"]) / r["pairs_evaluated"],
+        "method": "orc_count_in_support: sweep over dim 0, Fenwick tree over dim-1 ranks, |dx_e| < h_e in FP64",
+    }
+
+
 # ------------------------------------------------------------------ MI
 def mi_call_sequence(n_cols: int) -> list:
     """The columns of every combined_densities call of Mi::main, in its order
@@ -308,6 +323,11 @@ def run_mi(args) -> dict:
                       f"{np.mean(t2):.2f} s per call), extrapolated to all 4096 calls",
         }
         res["speedup_vs_cpu"] = total / sec
+        log("pairs inside the support (exact count on the host)")
+        pairs2 = [(i, j) for i in range(n_cols) for j in range(i + 1, n_cols)]
+        ins = orc.count_in_support_jobs(X, n_rows, [[c] for c in range(n_cols)], [[h]] * n_cols) + \
+            orc.count_in_support_jobs(X, n_rows, pairs2, [[h, h]] * len(pairs2))
+        res["support"] = support_fields(r, ins, sec)
         log("FP64 checks of sampled results")
         # four fixed entries and the matrix's smallest |MI| (the worst
         # conditioned: its terms cancel the most)
@@ -507,6 +527,7 @@ def run_entropy1d(args) -> dict:
                                **cpu_baseline_info(th, mask, why, "oracle restatement of densities_omp.cpp"),
                                "sample": "the full workload (one 1-D density, N=100000)"}
         res["speedup_vs_cpu"] = t / sec
+        res["support"] = support_fields(r, orc.count_in_support(x, n, [0], [h]), sec)
         ref = orc.density_fp64_window(x, n, [0], [h])
         errs = {"density (max over 100000 points)": float(np.max(np.abs(dens - ref) / ref)),
                 "entropy": rel(H, orc.entropy(ref))}

@@ -1093,31 +1093,31 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
   if (l < kStCount && ws.v != 0) atomicAdd(pairs + l, (unsigned long long)ws.v);
 }
 
-// Kernel variants of one Q: RAW (plain path, keys unused) or cell lists
-// with 32- or 64-bit keys.
-template <int Q>
-int density_occupancy_q(bool raw, bool wide, int* blocks_per_cu) {
-  const void* fn = raw    ? reinterpret_cast<const void*>(&k_density<Q, true, uint32_t>)
-                   : wide ? reinterpret_cast<const void*>(&k_density<Q, false, uint64_t>)
-                          : reinterpret_cast<const void*>(&k_density<Q, false, uint32_t>);
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, kBlock, 0);
+// One kernel variant: Q queries per lane at most, RAW (plain path, keys
+// unused) or cell lists with KeyT keys.  Each variant is instantiated in one
+// of the kde_density_*.hip files (so the slow ones compile in parallel).
+template <int Q, bool RAW, typename KeyT>
+int density_occupancy_v(int* blocks_per_cu) {
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocks_per_cu, reinterpret_cast<const void*>(&k_density<Q, RAW, KeyT>), kBlock, 0);
   return e == hipSuccess ? 0 : -1;
 }
 
-template <int Q>
-void launch_density_q(hipStream_t s, bool raw, bool wide, unsigned grid, const DJob* jobs, const DTile* tiles,
-                      const uint32_t* n_tiles, uint32_t* counter, const float* X, const float* scoord,
-                      const void* keys, const uint32_t* cstart, const uint32_t* idx, const float4* mom,
-                      const float* mom2, double* dens, unsigned long long* pairs) {
-  if (raw)
-    hipLaunchKernelGGL((k_density<Q, true, uint32_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
-                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, mom, mom2, dens, pairs);
-  else if (wide)
-    hipLaunchKernelGGL((k_density<Q, false, uint64_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
-                       counter, X, scoord, (const uint64_t*)keys, cstart, idx, mom, mom2, dens, pairs);
-  else
-    hipLaunchKernelGGL((k_density<Q, false, uint32_t>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles,
-                       counter, X, scoord, (const uint32_t*)keys, cstart, idx, mom, mom2, dens, pairs);
+template <int Q, bool RAW, typename KeyT>
+void launch_density_v(hipStream_t s, unsigned grid, const DJob* jobs, const DTile* tiles, const uint32_t* n_tiles,
+                      uint32_t* counter, const float* X, const float* scoord, const void* keys,
+                      const uint32_t* cstart, const uint32_t* idx, const float4* mom, const float* mom2,
+                      double* dens, unsigned long long* pairs) {
+  hipLaunchKernelGGL((k_density<Q, RAW, KeyT>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles, counter, X,
+                     scoord, (const KeyT*)keys, cstart, idx, mom, mom2, dens, pairs);
 }
+
+// explicit instantiation of one variant
+#define ENT_DENSITY_VARIANT(Q, RAW, KeyT)                                                                    \
+  template int density_occupancy_v<Q, RAW, KeyT>(int*);                                                     \
+  template void launch_density_v<Q, RAW, KeyT>(hipStream_t, unsigned, const DJob*, const DTile*,            \
+                                               const uint32_t*, uint32_t*, const float*, const float*,      \
+                                               const void*, const uint32_t*, const uint32_t*, const float4*, \
+                                               const float*, double*, unsigned long long*);
 
 }  // namespace ent

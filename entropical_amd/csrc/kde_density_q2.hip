@@ -1,9 +1,8 @@
-// kde_density_q2.hip -- k_density with at most 2 queries per lane.
+// kde_density_q2.hip -- k_density with at most 2 queries per lane (all variants).
 #include "kde_density.hpp"
 
 namespace ent {
-template int density_occupancy_q<2>(bool, bool, int*);
-template void launch_density_q<2>(hipStream_t, bool, bool, unsigned, const DJob*, const DTile*, const uint32_t*,
-                                   uint32_t*, const float*, const float*, const void*, const uint32_t*,
-                                   const uint32_t*, const float4*, const float*, double*, unsigned long long*);
+ENT_DENSITY_VARIANT(2, true, uint32_t)
+ENT_DENSITY_VARIANT(2, false, uint32_t)
+ENT_DENSITY_VARIANT(2, false, uint64_t)
 }  // namespace ent

@@ -1,9 +1,7 @@
-// kde_density_q4.hip -- k_density with at most 4 queries per lane.
+// kde_density_q4.hip -- k_density with at most 4 queries per lane: plain path and 32-bit cell keys.
 #include "kde_density.hpp"
 
 namespace ent {
-template int density_occupancy_q<4>(bool, bool, int*);
-template void launch_density_q<4>(hipStream_t, bool, bool, unsigned, const DJob*, const DTile*, const uint32_t*,
-                                   uint32_t*, const float*, const float*, const void*, const uint32_t*,
-                                   const uint32_t*, const float4*, const float*, double*, unsigned long long*);
+ENT_DENSITY_VARIANT(4, true, uint32_t)
+ENT_DENSITY_VARIANT(4, false, uint32_t)
 }  // namespace ent

@@ -1030,13 +1030,19 @@ void launch_tile_pack(hipStream_t s, const DJob* jobs, DTile* tiles, const uint3
   hipLaunchKernelGGL(k_tile_pack, dim3((unsigned)blocks), dim3(kBlock), 0, s, jobs, tiles, total);
 }
 
-// Density kernel variants for q = 1, 2, 4 queries per lane, each compiled
-// in its own file (kde_density_q*.hip).
+// Density kernel variants for q = 1, 2, 4 queries per lane, plain path or
+// cell lists with 32- or 64-bit keys (compiled in kde_density_*.hip).
+template <int Q>
+static int occupancy_q(bool raw, bool wide, int* bpc) {
+  if (raw) return density_occupancy_v<Q, true, uint32_t>(bpc);
+  return wide ? density_occupancy_v<Q, false, uint64_t>(bpc) : density_occupancy_v<Q, false, uint32_t>(bpc);
+}
+
 int density_occupancy(int q, bool raw, bool wide, int* blocks_per_cu) {
   switch (q) {
-    case 1: return density_occupancy_q<1>(raw, wide, blocks_per_cu);
-    case 2: return density_occupancy_q<2>(raw, wide, blocks_per_cu);
-    case 4: return density_occupancy_q<4>(raw, wide, blocks_per_cu);
+    case 1: return occupancy_q<1>(raw, wide, blocks_per_cu);
+    case 2: return occupancy_q<2>(raw, wide, blocks_per_cu);
+    case 4: return occupancy_q<4>(raw, wide, blocks_per_cu);
     default: return -1;
   }
 }
@@ -1045,9 +1051,18 @@ int launch_density(hipStream_t s, int q, bool raw, bool wide, unsigned grid, con
                    const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
                    const float* scoord, const void* keys, const uint32_t* cstart, const uint32_t* idx,
                    const float4* mom, const float* mom2, double* dens, unsigned long long* pairs) {
-#define ENT_LD(QQ)                                                                                        \
-  launch_density_q<QQ>(s, raw, wide, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart, idx, mom, \
-                       mom2, dens, pairs)
+#define ENT_LD(QQ)                                                                                              \
+  do {                                                                                                          \
+    if (raw)                                                                                                    \
+      launch_density_v<QQ, true, uint32_t>(s, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart, idx, \
+                                           mom, mom2, dens, pairs);                                             \
+    else if (wide)                                                                                              \
+      launch_density_v<QQ, false, uint64_t>(s, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart,   \
+                                            idx, mom, mom2, dens, pairs);                                       \
+    else                                                                                                        \
+      launch_density_v<QQ, false, uint32_t>(s, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart,   \
+                                            idx, mom, mom2, dens, pairs);                                       \
+  } while (0)
   switch (q) {
     case 1: ENT_LD(1); break;
     case 2: ENT_LD(2); break;

@@ -56,14 +56,14 @@ constexpr uint32_t kClaimStride = 1024;
 constexpr uint32_t kCounterWords = kClaimPartsMax * kClaimStride;
 static_assert(kClaimParts >= 1 && kClaimParts <= kClaimPartsMax, "claim partitions");
 int density_occupancy(int q, bool raw, bool wide, int* blocks_per_cu);
-// One queries-per-lane setting (instantiated in kde_density_q<Q>.hip).
-template <int Q>
-int density_occupancy_q(bool raw, bool wide, int* blocks_per_cu);
-template <int Q>
-void launch_density_q(hipStream_t s, bool raw, bool wide, unsigned grid, const DJob* jobs, const DTile* tiles,
-                      const uint32_t* n_tiles, uint32_t* counter, const float* X, const float* scoord,
-                      const void* keys, const uint32_t* cstart, const uint32_t* idx, const float4* mom,
-                      const float* mom2, double* dens, unsigned long long* pairs);
+// One density kernel variant (instantiated in the kde_density_*.hip files).
+template <int Q, bool RAW, typename KeyT>
+int density_occupancy_v(int* blocks_per_cu);
+template <int Q, bool RAW, typename KeyT>
+void launch_density_v(hipStream_t s, unsigned grid, const DJob* jobs, const DTile* tiles, const uint32_t* n_tiles,
+                      uint32_t* counter, const float* X, const float* scoord, const void* keys,
+                      const uint32_t* cstart, const uint32_t* idx, const float4* mom, const float* mom2,
+                      double* dens, unsigned long long* pairs);
 int launch_density(hipStream_t s, int q, bool raw, bool wide, unsigned grid, const DJob* jobs,
                    const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
                    const float* scoord, const void* keys, const uint32_t* cstart, const uint32_t* idx,

@@ -58,6 +58,8 @@ def lib() -> C.CDLL:
             "orc_negs_ref": (C.c_float, [P, SZ, C.c_int]),
             "orc_entropy": (C.c_double, [P, SZ]),
             "orc_loo_loglik": (None, [P, SZ, C.c_uint, P, P, C.c_uint, C.c_double, P]),
+            "orc_count_in_support": (C.c_uint64, [P, SZ, C.c_uint, P]),
+            "orc_count_in_support_jobs": (C.c_uint64, [P, SZ, C.c_uint, C.c_uint, P, P]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -251,6 +253,26 @@ def loo_loglik(pts_rowmajor, d, scale, hs, floor_p) -> np.ndarray:
     out = np.empty(hh.size, dtype=np.float64)
     lib().orc_loo_loglik(_p(x), n, d, _p(sc), _p(hh), hh.size, float(floor_p), _p(out))
     return out
+
+
+def count_in_support(coords, n_rows, i_cols, h) -> int:
+    """Ordered pairs (i, j), the self pair included, with |x_ie - x_je| < h_e
+    in every dim (d = 1, 2): the pairs whose Epanechnikov term is nonzero."""
+    c = _f32(coords).reshape(-1, n_rows)
+    packed = np.ascontiguousarray(c[list(i_cols)].reshape(-1))
+    hh = _f32(h)
+    return int(lib().orc_count_in_support(_p(packed), n_rows, len(i_cols), _p(hh)))
+
+
+def count_in_support_jobs(coords, n_rows, cols, h) -> int:
+    """count_in_support summed over jobs of one dimension d (cols: n_jobs x d
+    column indices, h: n_jobs x d bandwidths), OpenMP over the jobs."""
+    c = _f32(coords).reshape(-1)
+    cc = np.ascontiguousarray(np.asarray(cols, dtype=np.uint32))
+    hh = np.ascontiguousarray(np.asarray(h, dtype=np.float32))
+    n_jobs, d = cc.shape
+    assert hh.shape == cc.shape
+    return int(lib().orc_count_in_support_jobs(_p(c), n_rows, n_jobs, d, _p(cc), _p(hh)))
 
 
 # ---- reference harness (present only where the reference tree was built)

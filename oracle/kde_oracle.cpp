@@ -26,6 +26,7 @@
 //   orc_mi_ref / orc_te_ref / orc_autoent_ref / orc_kldiv_ref / orc_negs_ref
 //                             mi.cpp:53-56, transs.cpp:75-113,
 //                             autoent.cpp:52-57, kldiv.cpp:37-53, negs.cpp:37-47
+//   orc_count_in_support      pairs with a nonzero kernel term (measurement)
 //
 // Pinning (what checks each function against something other than itself)
 // is listed in tests/test_oracle.py.  Functions whose bit-level behaviour
@@ -38,6 +39,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <vector>
 #include <array>
 #include <omp.h>
@@ -563,6 +565,188 @@ double orc_entropy(const double* p, std::size_t n) {
   Neumaier s;
   for (std::size_t k = 0; k < n; ++k) s.add(std::log(p[k]));
   return -s.value() / (double)n;
+}
+
+// Pairs in the kernel's support: the number of ordered (query, point) pairs
+// (i, j), the self pair included, with |x_ie - x_je| < h_e in every dim --
+// exactly the pairs with a nonzero Epanechnikov term -- for d = 1 or 2
+// (packed column-major input of n points; differences in FP64).  O(n log n):
+// the points sorted by dim 0; a window of the points within h_0 along dim 0
+// slides over them; for d = 2 a Fenwick tree over the dim-1 ranks of the
+// window's points counts those within h_1.  (Measurement only: the share of
+// the evaluated pairs that lie inside the support.)
+uint64_t orc_count_in_support(const float* packed, std::size_t n, unsigned d, const float* h) {
+  if (n == 0 || d < 1 || d > 2) return 0;
+  std::vector<uint32_t> ord(n);
+  for (std::size_t i = 0; i < n; ++i) ord[i] = (uint32_t)i;
+  const float* x = packed;
+  std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return x[a] < x[b]; });
+  std::vector<double> xs(n);
+  for (std::size_t i = 0; i < n; ++i) xs[i] = (double)x[ord[i]];
+  const double h0 = (double)h[0];
+  uint64_t total = 0;
+  if (d == 1) {
+    std::size_t lo = 0, hi = 0;
+    for (std::size_t i = 0; i < n; ++i) {
+      while (xs[i] - xs[lo] >= h0) ++lo;
+      while (hi < n && xs[hi] - xs[i] < h0) ++hi;
+      total += hi - lo;
+    }
+    return total;
+  }
+  const float* y = packed + n;
+  const double h1 = (double)h[1];
+  std::vector<double> yv(n);
+  for (std::size_t i = 0; i < n; ++i) yv[i] = (double)y[i];
+  std::vector<double> uy(yv);
+  std::sort(uy.begin(), uy.end());
+  uy.erase(std::unique(uy.begin(), uy.end()), uy.end());
+  const std::size_t R = uy.size();
+  std::vector<uint32_t> rank(n);
+  for (std::size_t i = 0; i < n; ++i)
+    rank[i] = (uint32_t)(std::lower_bound(uy.begin(), uy.end(), yv[i]) - uy.begin());
+  std::vector<uint32_t> fen(R + 1, 0);
+  auto add = [&](uint32_t r, int v) {
+    for (std::size_t k = r + 1; k <= R; k += k & (~k + 1)) fen[k] += (uint32_t)v;
+  };
+  auto prefix = [&](std::size_t r) {  // points with rank < r
+    uint64_t c = 0;
+    for (std::size_t k = r; k > 0; k -= k & (~k + 1)) c += fen[k];
+    return c;
+  };
+  std::size_t lo = 0, hi = 0;
+  for (std::size_t i = 0; i < n; ++i) {
+    while (xs[i] - xs[lo] >= h0) add(rank[ord[lo++]], -1);
+    while (hi < n && xs[hi] - xs[i] < h0) add(rank[ord[hi++]], +1);
+    const double yi = yv[ord[i]];
+    // ranks r with uy[r] - yi > -h1 and uy[r] - yi < h1
+    const std::size_t a = (std::size_t)(std::upper_bound(uy.begin(), uy.end(), yi,
+                                                         [&](double v, double u) { return u - v > -h1; }) -
+                                        uy.begin());
+    const std::size_t b = (std::size_t)(std::lower_bound(uy.begin(), uy.end(), yi,
+                                                         [&](double u, double v) { return u - v < h1; }) -
+                                        uy.begin());
+    total += prefix(b) - prefix(a);
+  }
+  return total;
+}
+
+// The same summed over jobs of d = 1 or 2 columns of one column-major matrix
+// (job k: columns cols[k*d .. k*d + d), bandwidths h[k*d ..]), OpenMP over
+// the jobs.  The jobs of a batch share columns (the MI matrix: 2016 pairs of
+// 64), so each column's order by value and each (column, bandwidth)'s value
+// ranks and rank bounds are computed once; a 2-D job is then one sweep.
+namespace {
+struct SortedCol {  // dim-0 role: values in ascending order and the order
+  std::vector<uint32_t> ord;
+  std::vector<double> xs;
+};
+struct RankedCol {  // dim-1 role at bandwidth h: per point its value rank and
+  std::vector<uint32_t> rank, lo, hi;  // the ranks [lo, hi) within h of it
+  std::size_t R = 0;
+};
+SortedCol sort_col(const float* x, std::size_t n) {
+  SortedCol c;
+  c.ord.resize(n);
+  for (std::size_t i = 0; i < n; ++i) c.ord[i] = (uint32_t)i;
+  std::stable_sort(c.ord.begin(), c.ord.end(), [&](uint32_t a, uint32_t b) { return x[a] < x[b]; });
+  c.xs.resize(n);
+  for (std::size_t i = 0; i < n; ++i) c.xs[i] = (double)x[c.ord[i]];
+  return c;
+}
+RankedCol rank_col(const float* y, std::size_t n, double h1) {
+  RankedCol c;
+  std::vector<double> uy(n);
+  for (std::size_t i = 0; i < n; ++i) uy[i] = (double)y[i];
+  std::sort(uy.begin(), uy.end());
+  uy.erase(std::unique(uy.begin(), uy.end()), uy.end());
+  c.R = uy.size();
+  c.rank.resize(n);
+  c.lo.resize(n);
+  c.hi.resize(n);
+  for (std::size_t i = 0; i < n; ++i) {
+    const double yi = (double)y[i];
+    c.rank[i] = (uint32_t)(std::lower_bound(uy.begin(), uy.end(), yi) - uy.begin());
+    c.lo[i] = (uint32_t)(std::upper_bound(uy.begin(), uy.end(), yi, [&](double v, double u) { return u - v > -h1; }) -
+                         uy.begin());
+    c.hi[i] = (uint32_t)(std::lower_bound(uy.begin(), uy.end(), yi, [&](double u, double v) { return u - v < h1; }) -
+                         uy.begin());
+  }
+  return c;
+}
+uint64_t sweep_1d(const SortedCol& c, double h0) {
+  const std::size_t n = c.xs.size();
+  uint64_t total = 0;
+  std::size_t lo = 0, hi = 0;
+  for (std::size_t i = 0; i < n; ++i) {
+    while (c.xs[i] - c.xs[lo] >= h0) ++lo;
+    while (hi < n && c.xs[hi] - c.xs[i] < h0) ++hi;
+    total += hi - lo;
+  }
+  return total;
+}
+uint64_t sweep_2d(const SortedCol& c, double h0, const RankedCol& r) {
+  const std::size_t n = c.xs.size(), R = r.R;
+  std::vector<uint32_t> fen(R + 1, 0);
+  auto add = [&](uint32_t k0, uint32_t v) {
+    for (std::size_t k = k0 + 1; k <= R; k += k & (~k + 1)) fen[k] += v;
+  };
+  auto prefix = [&](std::size_t k0) {
+    uint64_t t = 0;
+    for (std::size_t k = k0; k > 0; k -= k & (~k + 1)) t += fen[k];
+    return t;
+  };
+  uint64_t total = 0;
+  std::size_t lo = 0, hi = 0;
+  for (std::size_t i = 0; i < n; ++i) {
+    while (c.xs[i] - c.xs[lo] >= h0) add(r.rank[c.ord[lo++]], ~0u);  // -1 mod 2^32
+    while (hi < n && c.xs[hi] - c.xs[i] < h0) add(r.rank[c.ord[hi++]], 1u);
+    const uint32_t q = c.ord[i];
+    total += prefix(r.hi[q]) - prefix(r.lo[q]);
+  }
+  return total;
+}
+}  // namespace
+
+uint64_t orc_count_in_support_jobs(const float* X, std::size_t n_rows, unsigned n_jobs, unsigned d,
+                                   const unsigned* cols, const float* h) {
+  if (d < 1 || d > 2 || n_rows == 0) return 0;
+  std::map<unsigned, std::size_t> s_id;
+  std::map<std::pair<unsigned, uint32_t>, std::size_t> r_id;
+  for (unsigned k = 0; k < n_jobs; ++k) {
+    s_id.emplace(cols[k * d], s_id.size());
+    if (d == 2) {
+      uint32_t hb;
+      std::memcpy(&hb, &h[k * d + 1], 4);
+      r_id.emplace(std::make_pair(cols[k * d + 1], hb), r_id.size());
+    }
+  }
+  std::vector<SortedCol> sc(s_id.size());
+  std::vector<RankedCol> rc(r_id.size());
+  std::vector<std::pair<unsigned, std::size_t>> sl(s_id.begin(), s_id.end());
+  std::vector<std::pair<std::pair<unsigned, uint32_t>, std::size_t>> rl(r_id.begin(), r_id.end());
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long i = 0; i < (long)sl.size(); ++i) sc[sl[i].second] = sort_col(X + (std::size_t)sl[i].first * n_rows, n_rows);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long i = 0; i < (long)rl.size(); ++i) {
+    float hv;
+    std::memcpy(&hv, &rl[i].first.second, 4);
+    rc[rl[i].second] = rank_col(X + (std::size_t)rl[i].first.first * n_rows, n_rows, (double)hv);
+  }
+  uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+  for (long k = 0; k < (long)n_jobs; ++k) {
+    const SortedCol& c = sc[s_id.at(cols[k * d])];
+    const double h0 = (double)h[k * d];
+    if (d == 1) {
+      total += sweep_1d(c, h0);
+    } else {
+      uint32_t hb;
+      std::memcpy(&hb, &h[k * d + 1], 4);
+      total += sweep_2d(c, h0, rc[r_id.at(std::make_pair(cols[k * d + 1], hb))]);
+    }
+  }
+  return total;
 }
 
 // Leave-one-out log-likelihood for a set of candidate isotropic scale

@@ -202,9 +202,7 @@ def test_packed_small_tiles(ctx, orc, monkeypatch, d):
     """Sparse tails give tiles of a few queries, which run packed two or four
     to a wave (16 or 32 lanes per tile).  Densities with packing and without
     it (ENT_TILE_PACK=0) both match FP64, and packing issues fewer lane
-    slots, so the packed path ran.  A packed 2-D tile sums its sub-tiles'
-    candidates in the order of a whole tile (64 per step, a compensated
-    merge every 32, the same origin): bit-identical densities."""
+    slots, so the packed path ran."""
     from entropical_amd import combined_densities
 
     r = np.random.default_rng(31 + d)
@@ -222,8 +220,6 @@ def test_packed_small_tiles(ctx, orc, monkeypatch, d):
     assert rel_err(single, ref) <= RTOL
     assert rel_err(packed, ref) <= RTOL
     assert slots_packed < slots_single
-    if d == 2:
-        np.testing.assert_array_equal(single, packed)
 
 
 @pytest.mark.gpu

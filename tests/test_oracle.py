@@ -438,3 +438,42 @@ def test_bf16_rounding_known_values():
     x = np.array([1.0, 1.00390625, 1.005859375, -3.14159265, 65504.0], np.float32)
     # 1.00390625 = 1 + 2^-8 is a tie between 1 and 1+2^-7 -> even (1.0)
     np.testing.assert_array_equal(M.bf16_round(x), [1.0, 1.0, 1.0078125, -3.140625, 65536.0])
+
+
+# ------------------------------------------------------- pairs in the support
+def _count_brute(x, cols, h):
+    d = np.ones((x.shape[1], x.shape[1]), dtype=bool)
+    for c, hh in zip(cols, h):
+        v = x[c].astype(np.float64)
+        d &= np.abs(v[:, None] - v[None, :]) < float(np.float32(hh))
+    return int(d.sum())
+
+
+def test_count_in_support_matches_brute_force(orc):
+    """Ordered pairs with a nonzero kernel term: the O(n log n) sweep equals a
+    brute-force count on random data with ties, for d = 1 and 2."""
+    r = np.random.default_rng(61)
+    for n in (1, 2, 50, 700):
+        x = r.standard_normal((3, n)).astype(np.float32)
+        x[1, ::5] = 0.5  # ties in dim 1
+        x[0, 1::7] = x[0, 0]  # ties in dim 0
+        for cols, h in (([0], [0.3]), ([1], [0.05]), ([0, 1], [0.4, 0.2]), ([2, 0], [0.1, 1.5])):
+            assert orc.count_in_support(x, n, cols, h) == _count_brute(x, cols, h), (n, cols, h)
+
+
+def test_count_in_support_boundary_and_jobs(orc):
+    """|x_i - x_j| = h exactly is outside the support (the kernel is zero
+    there); self pairs count; the job sum adds the jobs' counts."""
+    x = np.array([[0.0, 0.25, 0.5], [0.0, 0.0, 1.0]], np.float32)
+    # dim 0: |0 - 0.25| = 0.25 = h -> outside; 3 self pairs only
+    assert orc.count_in_support(x, 3, [0], [0.25]) == 3
+    # h slightly larger: (0, .25) and (.25, .5) both ways -> 3 + 4
+    assert orc.count_in_support(x, 3, [0], [0.2500001]) == 7
+    # 2-D: dim 1 separates point 2 (|0 - 1| = 1 >= 0.9) -> 3 + 2
+    assert orc.count_in_support(x, 3, [0, 1], [0.3, 0.9]) == 5
+    r = np.random.default_rng(62)
+    X = r.standard_normal((4, 300)).astype(np.float32)
+    cols = [[0, 1], [2, 3], [1, 3]]
+    h = [[0.2, 0.3], [0.5, 0.1], [0.25, 0.25]]
+    want = sum(_count_brute(X, c, hh) for c, hh in zip(cols, h))
+    assert orc.count_in_support_jobs(X, 300, cols, h) == want
