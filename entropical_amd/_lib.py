@@ -73,6 +73,7 @@ SIGNATURES = {
     "ent_ctx_last_sort": (C.c_int, [P, C.POINTER(C.c_uint32)]),
     "ent_ctx_last_moments": (C.c_int, [P, C.POINTER(C.c_uint64)]),
     "ent_ctx_last_tiles": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+    "ent_ctx_last_graph_replays": (C.c_int, [P, C.POINTER(C.c_uint32)]),
     "ent_column_stats_dev": (C.c_int, [P, P, C.c_size_t, C.c_size_t, P, P]),
     "ent_ctx_set_graph": (C.c_int, [P, C.c_int]),
     "ent_dev_alloc": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),

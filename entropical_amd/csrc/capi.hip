@@ -267,6 +267,16 @@ int ent_ctx_last_tiles(ent_ctx* ctx, uint64_t* out2) {
   return ENT_OK;
 }
 
+int ent_ctx_last_graph_replays(ent_ctx* ctx, uint32_t* out) {
+  CAPI_TRY(check_ctx(ctx));
+  if (!out) {
+    set_error("null argument");
+    return ENT_ERR_ARG;
+  }
+  *out = ctx->eng->last.graph_replays;
+  return ENT_OK;
+}
+
 int ent_dev_alloc(ent_ctx* ctx, size_t bytes, void** ptr) {
   CAPI_TRY(check_ctx(ctx));
   if (!ptr) {

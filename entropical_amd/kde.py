@@ -89,12 +89,14 @@ class Context:
         check(load().ent_ctx_last_moments(self.handle, mom))
         tl = (C.c_uint64 * 2)()
         check(load().ent_ctx_last_tiles(self.handle, tl))
+        gr = C.c_uint32()
+        check(load().ent_ctx_last_graph_replays(self.handle, C.byref(gr)))
         return {"density_ms": dm.value, "total_ms": tm.value, "pairs": pairs.value, "tiles": tiles.value,
                 "pairs_by_d": [int(v) for v in by_d[:3]], "lane_pairs": int(by_d[3]),
                 "radix_passes": int(srt[0]), "sort_bases": int(srt[1]),
                 "moment_pairs": int(mom[0]) + int(mom[1]), "moment_pairs_by_d": [int(mom[0]), int(mom[1])],
                 "moment_terms": int(mom[2]) + int(mom[3]), "moment_terms_by_d": [int(mom[2]), int(mom[3])],
-                "raw_tiles": int(tl[1])}
+                "raw_tiles": int(tl[1]), "graph_replays": int(gr.value)}
 
     # device memory -------------------------------------------------------
     def alloc(self, nbytes: int) -> int:

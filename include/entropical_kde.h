@@ -127,6 +127,11 @@ int ent_ctx_last_moments(ent_ctx* ctx, uint64_t* out4);
  * use coordinates scaled about the tile's centre.  No reference counterpart. */
 int ent_ctx_last_tiles(ent_ctx* ctx, uint64_t* out2);
 
+/* Captured graphs (launch sequences of a plan shape) replayed in the last
+ * density run instead of launched anew; 0 when every launch was captured
+ * or issued eagerly.  No reference counterpart. */
+int ent_ctx_last_graph_replays(ent_ctx* ctx, uint32_t* out);
+
 /* Device memory helpers on the context's device and stream. */
 int ent_dev_alloc(ent_ctx* ctx, size_t bytes, void** ptr);
 int ent_dev_free(ent_ctx* ctx, void* ptr);
