@@ -174,8 +174,9 @@ int ent_ctx_destroy(ent_ctx* ctx) {
 int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h, int sub) {
   CAPI_TRY(check_ctx(ctx));
   if (!(q == 1 || q == 2 || q == 4) || m0 < 1 || m0 > 64 || mr < 1 || mr > 8 || group_h < 1 || group_h > 64 ||
-      !(sub == 0 || sub == 16 || sub == 32)) {
-    set_error("invalid tuning (q in {1,2,4}, 1<=m0<=64, 1<=mr<=8, 1<=group_h<=64, sub in {0,16,32})");
+      !(sub == 0 || sub == 16 || sub == 32) || (sub != 0 && q != 1)) {
+    set_error("invalid tuning (q in {1,2,4}, 1<=m0<=64, 1<=mr<=8, 1<=group_h<=64, sub in {0,16,32}; "
+              "sub-tiles with q = 1)");
     return ENT_ERR_ARG;
   }
   for (ent::Engine* e : {ctx->eng, ctx->eng_na}) {

@@ -330,11 +330,14 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
       D.flags |= tuning.sub == 16 ? kJobSub16 : kJobSub32;
     }
     D.cells = cells;
+    D.cbase = table_total;
     if (cells + 1 <= kTableCellsPerPoint * (uint64_t)D.n + 4096) {
-      D.cbase = table_total;
       table_total += cells + 1;
     } else {
-      D.cbase = kNoTable;
+      // one entry per row of boxes: a binary search over one row's points
+      // instead of the job's
+      D.flags |= kJobRowTable;
+      table_total += cells / D.nb[0] + 1;
     }
     max_key = std::max<uint64_t>(max_key, cells - 1);
   }
@@ -360,7 +363,7 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
       D.mbase = 0;
       D.mcf = 1;
       D.mncb = D.nb[0];
-      if (!moments || D.cbase == kNoTable || (D.flags & kJobTri)) continue;
+      if (!moments || (D.flags & (kJobRowTable | kJobTri))) continue;
       if (D.d == 1 && D.nb[0] <= D.n) {
         D.flags |= kJobMoments;
         D.mbase = mom_total;

@@ -41,6 +41,36 @@ __device__ __forceinline__ uint32_t lower_bound_key(const KeyT* a, uint32_t n, u
   return lo;
 }
 
+// Sorted positions [a, b) of the points with keys klo..khi, which lie in one
+// row of boxes (row = key / nb0): read from the job's cell-start table, or
+// found by binary search within the row that its row-start table locates.
+template <typename KeyT>
+__device__ __forceinline__ void key_range(const DJob& J, const uint32_t* __restrict__ cstart,
+                                          const KeyT* __restrict__ kk, uint64_t row, uint64_t klo, uint64_t khi,
+                                          uint32_t& a, uint32_t& b) {
+  const uint32_t* t = cstart + J.cbase;
+  if (J.flags & kJobRowTable) {
+    const uint32_t r0 = t[row], r1 = t[row + 1];
+    a = r0 + lower_bound_key(kk + r0, r1 - r0, klo);
+    b = a + lower_bound_key(kk + a, r1 - a, khi + 1);
+  } else {
+    a = t[klo];
+    b = t[khi + 1];
+  }
+}
+// Sorted positions [a, b) of the rows row_a..row_b.
+__device__ __forceinline__ void rows_range(const DJob& J, const uint32_t* __restrict__ cstart, uint64_t row_a,
+                                           uint64_t row_b, uint32_t& a, uint32_t& b) {
+  const uint32_t* t = cstart + J.cbase;
+  if (J.flags & kJobRowTable) {
+    a = t[row_a];
+    b = t[row_b + 1];
+  } else {
+    a = t[row_a * J.nb[0]];
+    b = t[(row_b + 1) * J.nb[0]];
+  }
+}
+
 // Tiles whose queries span more than this many bandwidths along dim 0 (sparse
 // stretches; tiles stay inside one dim-0 group, up to 64 bandwidths wide)
 // evaluate u = (x_q - x_c) / h from the raw coordinates.  Narrower tiles use
@@ -379,18 +409,8 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
       uint32_t r1 = (D >= 2) ? lo[1] + (uint32_t)l % n1 : 0u;
       uint32_t r2 = (D == 3) ? lo[2] + (uint32_t)l / n1 : 0u;
       uint64_t row = (uint64_t)r2 * J.nb[1] + r1;
-      uint64_t klo = row * J.nb[0] + lo[0];
-      uint64_t khi = row * J.nb[0] + hi[0];
       uint32_t a, b;
-      if (J.cbase != kNoTable) {
-        const uint32_t* st = cstart + J.cbase;
-        a = st[klo];
-        b = st[khi + 1];
-      } else {
-        const KeyT* kk = keys + J.pbase;
-        a = lower_bound_key(kk, n, klo);
-        b = a + lower_bound_key(kk + a, n - a, khi + 1);
-      }
+      key_range(J, cstart, keys + J.pbase, row, row * J.nb[0] + lo[0], row * J.nb[0] + hi[0], a, b);
       my_lo = a;
       my_len = b - a;
     }
@@ -447,11 +467,9 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
       // row between the first and last candidate row (a superset).
       uint64_t row_a = (uint64_t)lo[2] * J.nb[1] + lo[1];
       uint64_t row_b = (uint64_t)hi[2] * J.nb[1] + hi[1];
-      const KeyT* kk = keys + J.pbase;
       if (l == 0) {
-        const bool tab = J.cbase != kNoTable;
-        uint32_t a = tab ? cstart[J.cbase + row_a * J.nb[0]] : lower_bound_key(kk, n, row_a * J.nb[0]);
-        uint32_t b = tab ? cstart[J.cbase + (row_b + 1) * J.nb[0]] : lower_bound_key(kk, n, (row_b + 1) * J.nb[0]);
+        uint32_t a, b;
+        rows_range(J, cstart, row_a, row_b, a, b);
         my_lo = a;
         my_len = b - a;
       }
@@ -621,23 +639,14 @@ __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __r
   const uint32_t n2 = (D == 3) ? (hi[2] - lo[2] + 1) : 1u;
   uint32_t nr = n1 * n2;
   uint32_t my_lo = 0, my_len = 0;
-  const bool tab = J.cbase != kNoTable;
   const KeyT* kk = keys + J.pbase;
   if (nr <= (uint32_t)W) {
     if ((uint32_t)j < nr) {
       const uint32_t r1 = lo[1] + (uint32_t)j % n1;
       const uint32_t r2 = (D == 3) ? lo[2] + (uint32_t)j / n1 : 0u;
       const uint64_t row = (uint64_t)r2 * J.nb[1] + r1;
-      const uint64_t klo = row * J.nb[0] + lo[0];
-      const uint64_t khi = row * J.nb[0] + hi[0];
       uint32_t a, b;
-      if (tab) {
-        a = cstart[J.cbase + klo];
-        b = cstart[J.cbase + khi + 1];
-      } else {
-        a = lower_bound_key(kk, n, klo);
-        b = a + lower_bound_key(kk + a, n - a, khi + 1);
-      }
+      key_range(J, cstart, kk, row, row * J.nb[0] + lo[0], row * J.nb[0] + hi[0], a, b);
       my_lo = a;
       my_len = b - a;
     }
@@ -647,9 +656,8 @@ __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __r
     if (j == 0) {
       const uint64_t row_a = (uint64_t)lo[2] * J.nb[1] + lo[1];
       const uint64_t row_b = (uint64_t)hi[2] * J.nb[1] + hi[1];
-      const uint32_t a = tab ? cstart[J.cbase + row_a * J.nb[0]] : lower_bound_key(kk, n, row_a * J.nb[0]);
-      const uint32_t b =
-          tab ? cstart[J.cbase + (row_b + 1) * J.nb[0]] : lower_bound_key(kk, n, (row_b + 1) * J.nb[0]);
+      uint32_t a, b;
+      rows_range(J, cstart, row_a, row_b, a, b);
       my_lo = a;
       my_len = b - a;
     }
@@ -786,25 +794,13 @@ __device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const f
   // the first and the last (a superset) on the sub-tile's first lane
   const uint32_t n1 = hi[1] - lo[1] + 1;
   const bool rows_fit = nq != 0 && n1 <= (uint32_t)W;
-  const bool tab = J.cbase != kNoTable;
-  const uint32_t* st = cstart + (tab ? J.cbase : 0);
-  const KeyT* kk = keys + J.pbase;
+  const uint32_t* st = cstart + J.cbase;  // the cell-start table of a job with moments
   const uint32_t r = lo[1] + (uint32_t)j;
   uint32_t ra = 0, rb = 0;
   if (rows_fit && (uint32_t)j < n1) {
-    const uint64_t klo = (uint64_t)r * J.nb[0] + lo[0];
-    const uint64_t khi = (uint64_t)r * J.nb[0] + hi[0];
-    if (tab) {
-      ra = st[klo];
-      rb = st[khi + 1];
-    } else {
-      ra = lower_bound_key(kk, n, klo);
-      rb = ra + lower_bound_key(kk + ra, n - ra, khi + 1);
-    }
+    key_range(J, cstart, keys + J.pbase, r, (uint64_t)r * J.nb[0] + lo[0], (uint64_t)r * J.nb[0] + hi[0], ra, rb);
   } else if (nq != 0 && !rows_fit && j == 0) {
-    const uint64_t ka = (uint64_t)lo[1] * J.nb[0], kb = (uint64_t)(hi[1] + 1) * J.nb[0];
-    ra = tab ? st[ka] : lower_bound_key(kk, n, ka);
-    rb = tab ? st[kb] : lower_bound_key(kk, n, kb);
+    rows_range(J, cstart, lo[1], hi[1], ra, rb);
   }
 
   // moment cells (rows [jlo, jlo + nir) x cells [clo, clo + nc)): inside the
@@ -1066,19 +1062,29 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
         if (pk & 3u) {
           const bool four = (pk & 3u) == 3u;
           if (d == 2) {
-            if (four) process_sub2<4, KeyT>(J, T, true, scoord, keys, cstart, idx, mom2, dens, L, ws);
-            else process_sub2<2, KeyT>(J, T, true, scoord, keys, cstart, idx, mom2, dens, L, ws);
+            // the variant with one query per lane runs packed 2-D tiles as
+            // sub-tiles (with moments); the others keep the lean pack path,
+            // so the sub-tile code does not add to their register pressure
+            if constexpr (Q == 1) {
+              if (four) process_sub2<4, KeyT>(J, T, true, scoord, keys, cstart, idx, mom2, dens, L, ws);
+              else process_sub2<2, KeyT>(J, T, true, scoord, keys, cstart, idx, mom2, dens, L, ws);
+            } else {
+              ws.add(kStPairs + 1, four ? process_pack<2, 4, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, ws)
+                                        : process_pack<2, 2, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, ws));
+            }
           } else {
             ws.add(kStPairs + 2, four ? process_pack<3, 4, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, ws)
                                       : process_pack<3, 2, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, ws));
           }
           continue;
         }
-        const uint32_t jf = __builtin_amdgcn_readfirstlane(J.flags);
-        if (d == 2 && (jf & (kJobSub16 | kJobSub32))) {
-          if (jf & kJobSub16) process_sub2<4, KeyT>(J, T, false, scoord, keys, cstart, idx, mom2, dens, L, ws);
-          else process_sub2<2, KeyT>(J, T, false, scoord, keys, cstart, idx, mom2, dens, L, ws);
-          continue;
+        if constexpr (Q == 1) {  // sub-tiles run with one query per lane
+          const uint32_t jf = __builtin_amdgcn_readfirstlane(J.flags);
+          if (d == 2 && (jf & (kJobSub16 | kJobSub32))) {
+            if (jf & kJobSub16) process_sub2<4, KeyT>(J, T, false, scoord, keys, cstart, idx, mom2, dens, L, ws);
+            else process_sub2<2, KeyT>(J, T, false, scoord, keys, cstart, idx, mom2, dens, L, ws);
+            continue;
+          }
         }
       }
       if (d == 1)

@@ -38,6 +38,9 @@ constexpr uint32_t kJobMoments = 16u;  // 1-D or 2-D job with a moment table: th
 constexpr uint32_t kJobSub16 = 32u;  // 2-D job whose tiles run as four sub-tiles of 16
                                      // queries, each with its own candidate stream
 constexpr uint32_t kJobSub32 = 64u;  // the same with two sub-tiles of 32 queries
+constexpr uint32_t kJobRowTable = 128u;  // cbase locates a row-start table (the first sorted
+                                         // position of every row of boxes, row = key / nb[0])
+                                         // instead of a cell-start table
 
 // Device job descriptor.  Points of job j, dim e, index k in [0, n):
 //   X[off[e] + k].
@@ -57,7 +60,8 @@ struct DJob {
   uint64_t out_off;  // first density slot in the output array
   double norm;       // prod_e(0.75/h_e) / n_rows
   uint64_t cells;    // prod_e nb[e]
-  uint64_t cbase;    // first entry of this job's cell-start table, or kNoTable
+  uint64_t cbase;    // first entry of this job's cell-start table (or row-start table,
+                     // kJobRowTable), or kNoTable
   uint64_t mbase;    // first entry of this job's box-moment table (kJobMoments)
   uint32_t mcf;      // 2-D moment cells: dim-0 boxes per cell
   uint32_t mncb;     // 2-D moment cells per row
@@ -66,7 +70,8 @@ struct DJob {
 };
 
 // Jobs whose cell count exceeds this multiple of their points (sparse grids,
-// heavy tails) locate candidate ranges by binary search instead of a table.
+// heavy tails) get a row-start table instead of a cell-start table and locate
+// candidate ranges by binary search within the row.
 constexpr uint64_t kNoTable = ~0ull;
 constexpr uint64_t kTableCellsPerPoint = 8;
 

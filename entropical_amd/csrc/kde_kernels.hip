@@ -735,8 +735,14 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
   uint32_t* st = start + (tab ? J.cbase : 0);
   uint32_t tiles = 0;  // tile starts among the lane's points
   const KeyT* kk = keys + J.pbase;
-  // Point p owns the cells (key[p-1], key[p]]; cells after the last key hold
-  // n.  Short runs are written by their own lane; runs longer than
+  // Row-start tables index rows of boxes (key / nb0), cell-start tables cells
+  // (the key itself).
+  const bool rowtab = (J.flags & kJobRowTable) != 0;
+  const KeyT nb0 = (KeyT)J.nb[0];
+  auto unit = [&](KeyT k) -> uint64_t { return rowtab ? (uint64_t)(k / nb0) : (uint64_t)k; };
+  const uint64_t units = rowtab ? J.cells / J.nb[0] : J.cells;
+  // Point p owns the units (unit(key[p-1]), unit(key[p])]; units after the
+  // last key hold n.  Short runs are written by their own lane; runs longer than
   // kLaneRun (empty stretches of a sparse grid) by the whole wave, one run
   // at a time, so one long run does not serialise on a single lane.
   constexpr uint64_t kLaneRun = 16;
@@ -754,8 +760,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
       if (p < J.n) {
         if (tile_start(J, prevk[ii], curk[ii], p)) ++tiles;
         if (tab) {
-          const uint64_t cur = curk[ii];
-          first = p == 0 ? 0 : (uint64_t)prevk[ii] + 1;
+          const uint64_t cur = unit(curk[ii]);
+          first = p == 0 ? 0 : unit(prevk[ii]) + 1;
           cnt = cur + 1 - first;
         }
       }
@@ -774,8 +780,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
         for (uint64_t e = (uint64_t)l; e < n; e += kWave) st[f + e] = v;
       }
       if (tab && J.n - 1 - p0 < (uint32_t)kWave) {  // this step holds the job's last point
-        const uint64_t tail = (uint64_t)kk[J.n - 1] + 1;
-        for (uint64_t cell = tail + (uint64_t)l; cell <= J.cells; cell += kWave) st[cell] = J.n;
+        const uint64_t tail = unit(kk[J.n - 1]) + 1;
+        for (uint64_t u = tail + (uint64_t)l; u <= units; u += kWave) st[u] = J.n;
       }
     }
   }

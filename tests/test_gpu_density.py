@@ -93,7 +93,7 @@ def test_all_points_identical(ctx):
 
 
 @pytest.mark.parametrize("tuning", [(1, 1, 1, 1), (2, 2, 1, 4), (4, 8, 2, 1), (4, 4, 3, 2), (1, 32, 4, 64, 16),
-                                    (2, 16, 2, 16, 32), (4, 8, 5, 2, 16)])
+                                    (1, 16, 2, 16, 32), (1, 8, 5, 2, 16)])
 def test_tunings_agree(ctx, orc, tuning):
     from entropical_amd import combined_densities
 
@@ -287,8 +287,10 @@ def test_sub_tiles_2d(ctx, orc, monkeypatch, sub, mr):
     tolerance; the sub-tiles' candidate windows are smaller, so fewer pairs
     are evaluated one by one than with whole tiles of the same shape; with
     moments on, some pairs come from cell moments."""
-    from entropical_amd import combined_densities
+    from entropical_amd import EntropicalError, combined_densities
 
+    with pytest.raises(EntropicalError, match="q = 1"):
+        ctx.set_tuning(2, 32, mr, 64, sub)  # sub-tiles run with one query per lane
     r = np.random.default_rng(151 + sub + mr)
     n = 80000
     a = r.standard_normal(n)
@@ -407,8 +409,9 @@ def test_key_width_change_replays_right_graph(ctx, orc, monkeypatch):
     cells) to 32-bit and back at unchanged radix pass counts, on one context
     with the same coordinate and density buffers: each run's captured
     launch sequence must be the one of its own key width.  Jobs this sparse
-    have no cell-start table, so the density kernel finds candidate ranges
-    by binary search over the keys, which a key of the other width breaks."""
+    have a row-start table instead of a cell-start table, so the density
+    kernel finds candidate ranges by binary search over a row's keys, which a
+    key of the other width breaks."""
     from entropical_amd import DeviceArray, batch_densities_dev
 
     r = np.random.default_rng(131)

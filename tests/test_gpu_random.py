@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 
 RTOL = 1e-5
 TUNINGS = [(1, 32, 2, 64), (2, 32, 1, 64), (1, 16, 2, 16), (4, 8, 1, 4), (2, 64, 2, 8), (1, 12, 3, 5),
-           (1, 32, 4, 64, 16), (2, 16, 4, 16, 32)]
+           (1, 32, 4, 64, 16), (1, 16, 4, 16, 32)]
 
 
 def sample(r, n):
