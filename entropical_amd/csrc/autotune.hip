@@ -155,7 +155,7 @@ int Engine::autotune(const float* X, const std::vector<JobSpec>& jobs, double* d
   // 2-D sub-tiles of 16 queries: finer rows pay there (a sub-tile's rows
   // bound its candidate window), wide dim-0 groups as for whole tiles
   if (std::any_of(jobs.begin(), jobs.end(), [](const JobSpec& j) { return j.d == 2; }))
-    for (int m0 : {16, 32})
+    for (int m0 : {32, 64})
       for (int mr : {2, 3, 4}) add(1, m0, mr, 64, 16);
   std::vector<std::pair<double, Tuning>> scored;
   for (const Tuning& t : grid) {

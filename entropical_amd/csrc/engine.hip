@@ -372,13 +372,18 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
         h_mlist.push_back(waves);
         waves += (D.nb[0] + kMomentBoxesPerWave - 1) / kMomentBoxesPerWave;
       } else if (D.d == 2) {
-        uint32_t cf = 1;  // the power of two nearest below m0 / 4
-        while (cf * 2 <= (uint32_t)std::max(1, tuning.m0 / 4)) cf *= 2;
+        // dim-0 boxes per cell: the power of two nearest below m0 / 4 (cells
+        // about h/4 wide); jobs with sub-tiles merge their cells by rows, so
+        // narrower cells (m0 / 8, about h/8) cost them little and leave fewer
+        // pairs to the pair loop
+        const bool sub = (D.flags & (kJobSub16 | kJobSub32)) != 0;
+        uint32_t cf = 1;
+        while (cf * 2 <= (uint32_t)std::max(1, tuning.m0 / (sub ? 8 : 4))) cf *= 2;
         const uint32_t ncb = (D.nb[0] + cf - 1) / cf;
         const uint64_t cells = (uint64_t)ncb * D.nb[1];
         // cells hold 4 points on average at least (at most 9 B of table per
-        // point); jobs with sub-tiles, whose rows are finer, 1 point
-        const uint64_t per = (D.flags & (kJobSub16 | kJobSub32)) ? 1 : 4;
+        // point); jobs with sub-tiles, whose cells are smaller, 1 point
+        const uint64_t per = sub ? 1 : 4;
         if (per * cells > D.n) continue;
         D.flags |= kJobMoments;
         D.mbase = mom2_total;

@@ -832,7 +832,6 @@ __device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const f
   }
   const uint32_t pts = sub_sum<W>(inr ? m2 - m1 : 0u);
   const bool use_m = nir != 0 && pts >= 4u * nir * nc;
-  const uint32_t cells = use_m ? nir * nc : 0u;
 
   // range table: [ra, rb), or [ra, m1) and [m2, rb) for an interior row
   uint32_t a_len = rb - ra, b_pos = rb, b_len = 0;
@@ -850,49 +849,63 @@ __device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const f
   wave_sync();
 
   float S[1] = {0.0f}, C[1] = {0.0f};
-  // moment cells: W per sub-tile and round, lane j staging cell
-  // round * W + j of its sub-tile (padding cells are zero)
-  const uint32_t rounds = wave_max_u32((cells + W - 1) / W);
-  if (rounds != 0) {
+  // Moment cells, merged by rows: lane j of a sub-tile with moment cells sums
+  // the nc cells of its interior row jlo + j (j < nir) into one cell about the
+  // origin of the row's middle interior cell (short offsets keep the
+  // polynomial's terms small against their sum).  A cell's dim-0 moments move to
+  // an origin delta further on as N'0 = N0, N'1 = N1 - 2 k0 delta N0,
+  // N'2 = N2 + delta N1 - k0 delta^2 N0 (for each dim-1 index); the merged cell
+  // holds only points inside every query's support, so each query adds one
+  // polynomial term per interior row instead of one per cell.
+  const bool mrow = use_m && (uint32_t)j < nir;
+  if (wave_max_u32(mrow ? 1u : 0u) != 0) {
     const float k0 = J.hinv[0] * J.hinv[0], k1 = J.hinv[1] * J.hinv[1];
-    const int cs0 = s * kCellStride;
-    for (uint32_t rd = 0; rd < rounds; ++rd) {
-      const uint32_t k = rd * W + (uint32_t)j;
-      float N[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      float o0 = 0.f, o1 = 0.f;
-      if (k < cells) {
-        const uint32_t ri = k / nc;
-        const uint32_t row = jlo + ri, cb = clo + (k - ri * nc);
-        const float* srcm = mom2 + (J.mbase + (uint64_t)row * J.mncb + cb) * 9;
+    float M[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float mo0 = 0.f, mo1 = 0.f;
+    if (mrow) {
+      const uint32_t row = jlo + (uint32_t)j;
+      mo0 = fmaf((float)((clo + nc / 2) * J.mcf), J.bs[0], J.mn[0]);
+      mo1 = fmaf((float)row, J.bs[1], J.mn[1]);
+      const float* srcm = mom2 + (J.mbase + (uint64_t)row * J.mncb + clo) * 9;
+#pragma unroll 2
+      for (uint32_t c = 0; c < nc; ++c) {
+        float N[9];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) N[i] = srcm[i];
-        o0 = fmaf((float)(cb * J.mcf), J.bs[0], J.mn[0]);
-        o1 = fmaf((float)row, J.bs[1], J.mn[1]);
-      }
-      const uint32_t left = cells > rd * W ? min(cells - rd * W, (uint32_t)W) : 0u;
-      const uint32_t cnt8 = (wave_max_u32(left) + 7u) & ~7u;
-      wave_sync();
-      L.gcell[0][cs0 + j] = make_float4(N[0], N[1], N[2], N[3]);
-      L.gcell[1][cs0 + j] = make_float4(N[4], N[5], N[6], N[7]);
-      L.gcell[2][cs0 + j] = make_float4(N[8], o0, o1, 0.f);
-      wave_sync();
-      for (uint32_t i8 = 0; i8 < cnt8; i8 += 8) {
-        float part = 0.0f;
-#pragma unroll 1
-        for (int i = 0; i < 8; ++i) {
-          // a = (N00, N01, N02, N10), b = (N11, N12, N20, N21), c = (N22, r0, r1, -)
-          const int ci = cs0 + (int)i8 + i;
-          const float4 a = L.gcell[0][ci], b = L.gcell[1][ci], c = L.gcell[2][ci];
-          const float d0 = xq[0] - c.y, d1 = xq[1] - c.z;
-          const float a0 = __builtin_fmaf(-(k0 * d0), d0, 1.0f);
-          const float bb0 = __builtin_fmaf(-(k1 * d1), d1, 1.0f);
-          const float w0 = __builtin_fmaf(a0, a.x, __builtin_fmaf(d0, a.w, b.z));
-          const float w1 = __builtin_fmaf(a0, a.y, __builtin_fmaf(d0, b.x, b.w));
-          const float w2 = __builtin_fmaf(a0, a.z, __builtin_fmaf(d0, b.y, c.x));
-          part += __builtin_fmaf(bb0, w0, __builtin_fmaf(d1, w1, w2));
+        for (int i = 0; i < 9; ++i) N[i] = srcm[9 * c + i];
+        const float dl = mo0 - fmaf((float)((clo + c) * J.mcf), J.bs[0], J.mn[0]);
+        const float t1 = -2.0f * k0 * dl, t2 = -k0 * dl * dl;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          M[q] += N[q];
+          M[3 + q] += __builtin_fmaf(t1, N[q], N[3 + q]);
+          M[6 + q] += __builtin_fmaf(t2, N[q], __builtin_fmaf(dl, N[3 + q], N[6 + q]));
         }
-        kahan_add(S[0], C[0], part);
       }
+    }
+    const int cs0 = s * (W + 1);
+    wave_sync();
+    L.gcell[0][cs0 + j] = make_float4(M[0], M[1], M[2], M[3]);
+    L.gcell[1][cs0 + j] = make_float4(M[4], M[5], M[6], M[7]);
+    L.gcell[2][cs0 + j] = make_float4(M[8], mo0, mo1, 0.f);
+    wave_sync();
+    // every query of a sub-tile adds its merged rows (padding rows are zero)
+    const uint32_t rows = wave_max_u32(use_m ? nir : 0u);
+    for (uint32_t i8 = 0; i8 < rows; i8 += 8) {
+      float part = 0.0f;
+#pragma unroll 1
+      for (uint32_t i = i8; i < i8 + 8 && i < rows; ++i) {
+        // a = (N00, N01, N02, N10), b = (N11, N12, N20, N21), c = (N22, r0, r1, -)
+        const int ci = cs0 + (int)i;
+        const float4 a = L.gcell[0][ci], b = L.gcell[1][ci], c = L.gcell[2][ci];
+        const float d0 = xq[0] - c.y, d1 = xq[1] - c.z;
+        const float a0 = __builtin_fmaf(-(k0 * d0), d0, 1.0f);
+        const float bb0 = __builtin_fmaf(-(k1 * d1), d1, 1.0f);
+        const float w0 = __builtin_fmaf(a0, a.x, __builtin_fmaf(d0, a.w, b.z));
+        const float w1 = __builtin_fmaf(a0, a.y, __builtin_fmaf(d0, b.x, b.w));
+        const float w2 = __builtin_fmaf(a0, a.z, __builtin_fmaf(d0, b.y, c.x));
+        part += __builtin_fmaf(bb0, w0, __builtin_fmaf(d1, w1, w2));
+      }
+      kahan_add(S[0], C[0], part);
     }
   }
 
@@ -962,7 +975,7 @@ __device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const f
   const bool first = j == 0;
   uint64_t cp = first ? (uint64_t)nq * (total + mp) : 0ull;
   uint64_t mpp = first ? (uint64_t)nq * mp : 0ull;
-  uint64_t mtt = first ? (uint64_t)nq * cells : 0ull;
+  uint64_t mtt = first ? (uint64_t)nq * (use_m ? nir : 0u) : 0ull;
   uint64_t sc = 0, sm = 0, st2 = 0;
 #pragma unroll
   for (int g = 0; g < G; ++g) {

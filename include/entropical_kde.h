@@ -282,7 +282,7 @@ int ent_hestimate_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_
 /* Tile-shape autotuner (no reference counterpart; the reference fixes its
  * tile at compile time, densities_cuda.cu:14).  Times every setting of the
  * grid q in {1,2,4}, m0 in {8,16,32}, mr in {1,2}, group_h in {4,16,64} with
- * whole 2-D tiles, and for batches with 2-D jobs also q = 1, m0 in {16,32},
+ * whole 2-D tiles, and for batches with 2-D jobs also q = 1, m0 in {32,64},
  * mr in {2,3,4}, group_h = 64 with 2-D sub-tiles of 16, on a
  * subsample of the job set (the four fastest re-timed on the whole set),
  * keeps the fastest on the context (and returns it in tuning5 = (q, m0, mr,

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 GRID_Q, GRID_M0, GRID_MR, GRID_GH = (1, 2, 4), (8, 16, 32), (1, 2), (4, 16, 64)
 # 2-D sub-tiles of 16 (batches with 2-D jobs)
-SUB_Q, SUB_M0, SUB_MR = (1,), (16, 32), (2, 3, 4)
+SUB_Q, SUB_M0, SUB_MR = (1,), (32, 64), (2, 3, 4)
 
 
 def in_grid(t, has_2d=True):

@@ -295,7 +295,9 @@ def test_sub_tiles_2d(ctx, orc, monkeypatch, sub, mr):
     n = 80000
     a = r.standard_normal(n)
     x = np.stack([a, 0.6 * a + 0.8 * r.standard_normal(n)]).astype(np.float32)
-    h = [0.12, 0.15]
+    # bandwidths for which the job's moment table (cells h/8 wide, rows h/mr
+    # high) has fewer cells than points, so it is built
+    h = [0.2, 0.25]
     ref = orc.density_fp64_window(x, n, [0, 1], h)
     got, st = {}, {}
     try:
