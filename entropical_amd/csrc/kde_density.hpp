@@ -1069,7 +1069,7 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
       const DTile T = tiles[t];
       const uint32_t pk = __builtin_amdgcn_readfirstlane(T.pack);
       if (pk == kTileSkip) continue;
-      const DJob J = jobs[T.job];
+      const DJob& J = jobs[__builtin_amdgcn_readfirstlane(T.job)];  // uniform: scalar loads
       const uint32_t d = __builtin_amdgcn_readfirstlane(J.d);
       if constexpr (!RAW) {
         if (pk & 3u) {
