@@ -145,7 +145,9 @@ def features_from_stats(st: np.ndarray, n: int, corr: Optional[np.ndarray] = Non
     f[12] = np.clip(kurt, -2.0, 20.0).mean() / 10.0
     f[13] = iqr.mean()
     f[14] = (q[8] - q[0]).mean()
-    f[15] = ((skew ** 2 + 1.0) / (kurt + 3.0)).mean()
+    # bimodality coefficient; a constant dimension (fourth moment 0) counts 0
+    m4 = kurt + 3.0
+    f[15] = np.divide(skew ** 2 + 1.0, m4, out=np.zeros_like(m4), where=m4 > 0).mean()
     if d > 1:
         off = np.abs(corr[~np.eye(d, dtype=bool)])
         f[16] = off.mean()
