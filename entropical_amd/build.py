@@ -22,7 +22,8 @@ BUILD = ROOT / "_build"
 LIB = PKG / "libentropical_kde.so"
 ARCH = os.environ.get("ENT_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kde_density_q4.hip", "kde_density_q4w.hip", "kde_density_q2.hip", "kde_density_q1.hip", "kde_kernels.hip",
+HIP_SOURCES = ["kde_density_q4.hip", "kde_density_q4w.hip", "kde_density_q2.hip", "kde_density_q1.hip",
+               "kde_density_q1s3.hip", "kde_kernels.hip",
                "engine.hip", "autotune.hip", "capi.hip", "reductions.hip", "drivers.hip",
                "mlp.hip", "noalloc.hip", "colstats.hip"]
 HOST_SOURCES = ["densities_shim.cpp", "xtc.cpp"]

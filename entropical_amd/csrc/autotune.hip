@@ -2,8 +2,8 @@
 //
 // Search space: maximum queries per lane q (tile = 64*q queries), dim-0 boxes
 // per bandwidth m0, row boxes per bandwidth mr, tile extent along dim 0 in
-// bandwidths group_h, and for batches with 2-D jobs the sub-tile size of
-// their tiles.  Every setting of the grid is timed on a subsample of the
+// bandwidths group_h, and for batches with 2-D or 3-D jobs the sub-tile size
+// of their tiles.  Every setting of the grid is timed on a subsample of the
 // batch (after one untimed run, so graph capture and workspace growth are
 // excluded); the four fastest are timed again on the whole batch.
 #include <hip/hip_runtime.h>
@@ -36,7 +36,7 @@ std::string signature(const std::vector<JobSpec>& jobs, const std::vector<double
     for (uint32_t e = 0; e < s.d; ++e) lsum[s.d - 1][e] += std::log2(std::max(range_h[3 * j + e], 1e-3));
   }
   std::ostringstream os;
-  os << "v2;";  // version of the search space (entries of other versions never match)
+  os << "v3;";  // version of the search space (entries of other versions never match)
   for (int d = 0; d < 3; ++d) {
     double p = (double)pts[d];
     double r = p > 0 ? std::round(p / std::pow(10.0, std::floor(std::log10(p)) - 1)) *
@@ -157,6 +157,11 @@ int Engine::autotune(const float* X, const std::vector<JobSpec>& jobs, double* d
   if (std::any_of(jobs.begin(), jobs.end(), [](const JobSpec& j) { return j.d == 2; }))
     for (int m0 : {32, 64})
       for (int mr : {2, 3, 4}) add(1, m0, mr, 64, 16);
+  // 3-D sub-tiles of 16 queries (rows h or h/2 high; the batch's 2-D jobs
+  // then run whole tiles)
+  if (std::any_of(jobs.begin(), jobs.end(), [](const JobSpec& j) { return j.d == 3; }))
+    for (int m0 : {16, 32})
+      for (int mr : {1, 2}) add(1, m0, mr, 64, 16);
   std::vector<std::pair<double, Tuning>> scored;
   for (const Tuning& t : grid) {
     double ms = 0.0;

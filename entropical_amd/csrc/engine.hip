@@ -322,10 +322,12 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
       }
     }
     D.group = (uint32_t)std::max(1, tuning.group_h * tuning.m0);
-    // 2-D jobs with sub-tiles: tiles of up to 64 queries (four of 16 or two
-    // of 32 side by side in one wave)
+    // 2-D jobs, and 3-D jobs with rows at least h/2 high (a sub-tile's
+    // candidate rows then fit its lanes' two range entries each), with
+    // sub-tiles: tiles of up to 64 queries (four of 16 or two of 32 side by
+    // side in one wave)
     D.tq = (uint32_t)(kWave * tuning.q);
-    if (D.d == 2 && (tuning.sub == 16 || tuning.sub == 32)) {
+    if ((D.d == 2 || (D.d == 3 && tuning.mr <= 2)) && (tuning.sub == 16 || tuning.sub == 32)) {
       D.tq = kWave;
       D.flags |= tuning.sub == 16 ? kJobSub16 : kJobSub32;
     }
@@ -420,6 +422,9 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   const int npass_full = passes_for(max_key);
   const bool wide = bits > 32;
   keys_wide = wide;
+  sub3 = false;
+  for (const DJob& D : h_jobs)
+    if (D.d == 3 && (D.flags & (kJobSub16 | kJobSub32))) sub3 = true;
 
   // Two-stage sort: jobs with the same dim-0 window (offset, length) and
   // dim-0 bandwidth have the same dim-0 boxes; one stable sort of that window
@@ -799,7 +804,8 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
   (void)max_tiles;
   int bpc = 0;
   const bool wide = !raw && keys_wide;
-  if (density_occupancy(tuning.q, raw, wide, &bpc) != 0 || bpc <= 0) {
+  const bool s3 = !raw && tuning.q == 1 && sub3;
+  if (density_occupancy(tuning.q, raw, wide, s3, &bpc) != 0 || bpc <= 0) {
     set_error("unsupported queries-per-lane setting");
     return kErrUnsupported;
   }
@@ -812,7 +818,7 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
   // memset node) inside the same graph as the persistent launch
   auto issue = [&] {
     launch_fill_u32(stream_, counter, 0u, kCounterWords);
-    rc = launch_density(stream_, tuning.q, raw, wide, grid, (const DJob*)d_jobs.p, (const DTile*)d_tiles.p,
+    rc = launch_density(stream_, tuning.q, raw, wide, s3, grid, (const DJob*)d_jobs.p, (const DTile*)d_tiles.p,
                         n_tiles, counter, X, raw ? nullptr : (const float*)d_scoord.p,
                         raw ? nullptr : d_keys[sorted_buf].p, raw ? nullptr : (const uint32_t*)d_cellstart.p,
                         raw ? nullptr : (const uint32_t*)d_idx[sorted_buf].p,
@@ -827,6 +833,7 @@ int Engine::launch_density_common(bool raw, const float* X, double* dens, uint64
     PlanKey key = plan_key(X, dens, (uint32_t)h_jobs.size(), NC, total_points, sorted_buf, false);
     key.grid = (int)grid;
     key.raw = wide ? 2 : 0;  // key width selects the kernel variant
+    key.variant = s3 ? 1 : 0;
     ENT_TRY(run_captured(dens_graphs_, key, issue));
   }
   if (rc != 0) {

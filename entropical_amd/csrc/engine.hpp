@@ -182,6 +182,7 @@ class Engine {
   uint64_t total_points = 0;
   int sorted_buf = 0;  // which of d_keys/d_idx holds the sorted result
   bool keys_wide = false;  // cell keys stored as uint64 (else uint32)
+  bool sub3 = false;       // 3-D jobs with sub-tiles: the q = 1 variant with 3-D sub-tiles
 };
 
 }  // namespace ent

@@ -82,13 +82,14 @@ __device__ __forceinline__ void rows_range(const DJob& J, const uint32_t* __rest
 constexpr float kLocalSpan = 32.0f;
 
 // LDS of one wave.  Whole tiles stage 64 candidates in cand and 2-D moment
-// cells in cand + cell.  Sub-tiles of 2-D tiles (process_sub2) stage their
+// cells in cand + cell.  Sub-tiles (process_subt) of 2-D tiles stage their
 // moment cells in gcell (sub-tile s from kSubCellStride * s) and 64
-// candidates each in gcand (sub-tile s from kSubCandStride * s): the strides
-// are one float4 longer than the data, so the addresses that the sub-tiles'
-// lane groups read with one instruction fall on different LDS banks.
+// candidates each in gcand (sub-tile s from kSubCandStride * s; 3-D tiles 32
+// candidates as float4): the strides are one float4 longer than the data, so
+// the addresses that the sub-tiles' lane groups read with one instruction
+// fall on different LDS banks.
 constexpr int kSubCellStride = kWave / 4 + 1;  // float4 per sub-tile of 16 (a 32-lane one: 2x)
-constexpr int kSubCandStride = kWave / 2 + 1;  // float4 per sub-tile (64 2-D candidates)
+constexpr int kSubCandStride = kWave / 2 + 1;  // float4 per sub-tile (64 2-D or 32 3-D candidates)
 struct WaveLds {
   union {
     struct {
@@ -99,7 +100,7 @@ struct WaveLds {
     float4 gcand[4 * kSubCandStride];
   };
   uint2 rng[2 * kWave];  // candidate ranges (first flattened index, first position): packed
-                         // tiles W per sub-tile, 2-D sub-tiles 2W per sub-tile
+                         // tiles W per sub-tile, sub-tiles (process_subt) 2W per sub-tile
 };
 
 // Conservative box range [lo, hi] covering [a - h, b + h] along one dim.
@@ -727,36 +728,42 @@ __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __r
   return pr;
 }
 
-// Densities of a 2-D tile run as G sub-tiles side by side, W = 64 / G lanes
-// and one query per lane each: a packed tile (sub-tile offsets in T.pack) or
-// a tile of a kJobSub16 / kJobSub32 job split into runs of W consecutive
-// queries (a short tile leaves its last sub-tiles empty).  Fewer queries
+// Densities of a D-dim tile (D = 2, 3) run as G sub-tiles side by side,
+// W = 64 / G lanes and one query per lane each: a packed tile (sub-tile
+// offsets in T.pack) or a tile of a kJobSub16 / kJobSub32 job split into runs
+// of W consecutive queries (a short tile leaves its last sub-tiles empty).  Fewer queries
 // share a candidate stream than in a whole tile, so the candidates that lie
 // in the support of only some of them -- the margin between the union and
 // the intersection of their supports, which grows with the extent of the
 // query set -- are fewer.  Each sub-tile has its own bounding box, moment
-// cells (jobs with a moment table: the cells of its interior rows and
-// interior dim-0 span, as for a whole tile), candidate ranges (per candidate
-// row one range, or two around the moment cells, in the wave's LDS range
-// table: entries 2Ws + 2j and 2Ws + 2j + 1 for row j of sub-tile s) and
-// candidate stream.  The wave steps through the longest stream 64
-// candidates per sub-tile at a time, each lane loading G candidates of its
-// sub-tile per step.
-template <int G, typename KeyT>
-__device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const float* __restrict__ scoord,
+// cells (2-D jobs with a moment table: the cells of its interior rows and
+// interior dim-0 span, as for a whole tile), candidate ranges (two entries
+// per lane in the wave's LDS range table, 2Ws + 2j and 2Ws + 2j + 1 for lane
+// j of sub-tile s: in 2-D candidate row j, split in two around the moment
+// cells; in 3-D the candidate rows j and j + W, a row being one (dim-1,
+// dim-2) pair of boxes) and candidate stream.  The wave steps through the
+// longest stream 64 candidates (3-D: 32) per sub-tile at a time, each lane
+// loading G (G / 2) candidates of its sub-tile per step.
+template <int D, int G, typename KeyT>
+__device__ void process_subt(const DJob& J, const DTile& T, bool packed, const float* __restrict__ scoord,
                              const KeyT* __restrict__ keys, const uint32_t* __restrict__ cstart,
                              const uint32_t* __restrict__ idx, const float* __restrict__ mom2,
                              double* __restrict__ dens, WaveLds& L, WaveStats& ws) {
   constexpr int W = kWave / G;
-  constexpr int kNC = kWave;  // candidates per sub-tile and step
+  // candidates per sub-tile and step, and per lane (3-D: half as many, so
+  // fewer coordinates are in flight per lane)
+  constexpr int kNC = D == 3 ? kWave / 2 : kWave;
+  constexpr int kLd = kNC / W;
   constexpr int kCellStride = W + 1;
   static_assert(G * kCellStride <= 4 * kSubCellStride, "moment-cell stage");
   const int l = lane_id();
   const int s = l / W, j = l % W;
   const uint32_t n = J.n;
+  static_assert(D == 2 || D == 3, "sub-tiles of 2-D and 3-D tiles");
   const float* src0 = scoord + 3 * J.pbase;
   const float* src1 = src0 + n;
-  const float hinv[3] = {J.hinv[0], J.hinv[1], 1.0f};
+  const float* src2 = src1 + n;
+  const float hinv[3] = {J.hinv[0], J.hinv[1], D == 3 ? J.hinv[2] : 1.0f};
 
   // this lane's sub-tile and query; idle lanes repeat their sub-tile's first
   // query, the lanes of an empty sub-tile the tile's first (they get no
@@ -773,15 +780,15 @@ __device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const f
   const uint32_t nq = b1 - b0;
   const bool qv = (uint32_t)j < nq;
   const uint32_t p = T.p0 + (nq == 0 ? 0u : b0 + (qv ? (uint32_t)j : 0u));
-  const float xq[2] = {src0[p], src1[p]};
+  const float xq[3] = {src0[p], src1[p], D == 3 ? src2[p] : 0.0f};
   const uint32_t slot = idx[J.pbase + p];
 
   // bounding box (its centre is the origin of the local coordinates) and the
   // boxes of the candidate window
-  float bmin[2], bmax[2], o[2];
-  uint32_t lo[2], hi[2];
+  float bmin[3], bmax[3], o[3];
+  uint32_t lo[3], hi[3];
 #pragma unroll
-  for (int e = 0; e < 2; ++e) {
+  for (int e = 0; e < D; ++e) {
     bmin[e] = sub_min<W>(xq[e]);
     bmax[e] = sub_max<W>(xq[e]);
     o[e] = 0.5f * (bmin[e] + bmax[e]);
@@ -789,18 +796,37 @@ __device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const f
   }
   const bool exact = wave_max((bmax[0] - bmin[0]) * J.hinv[0]) > kLocalSpan;
 
-  // this lane's candidate row r and its range [ra, rb) of sorted positions;
-  // with more candidate rows than lanes, one range over every row between
-  // the first and the last (a superset) on the sub-tile's first lane
+  // this lane's candidate row r and its range [ra, rb) of sorted positions
+  // (3-D: also row r + W's, [ra2, rb2)); with more candidate rows than the
+  // lanes take, one range over every row between the first and the last (a
+  // superset) on the sub-tile's first lane
   const uint32_t n1 = hi[1] - lo[1] + 1;
-  const bool rows_fit = nq != 0 && n1 <= (uint32_t)W;
+  const uint32_t nr = D == 3 ? n1 * (hi[2] - lo[2] + 1) : n1;
+  const bool rows_fit = nq != 0 && nr <= (uint32_t)(D == 3 ? 2 * W : W);
   const uint32_t* st = cstart + J.cbase;  // the cell-start table of a job with moments
   const uint32_t r = lo[1] + (uint32_t)j;
-  uint32_t ra = 0, rb = 0;
-  if (rows_fit && (uint32_t)j < n1) {
-    key_range(J, cstart, keys + J.pbase, r, (uint64_t)r * J.nb[0] + lo[0], (uint64_t)r * J.nb[0] + hi[0], ra, rb);
-  } else if (nq != 0 && !rows_fit && j == 0) {
-    rows_range(J, cstart, lo[1], hi[1], ra, rb);
+  const KeyT* kk = keys + J.pbase;
+  uint32_t ra = 0, rb = 0, ra2 = 0, rb2 = 0;
+  if constexpr (D == 2) {
+    if (rows_fit && (uint32_t)j < n1) {
+      key_range(J, cstart, kk, r, (uint64_t)r * J.nb[0] + lo[0], (uint64_t)r * J.nb[0] + hi[0], ra, rb);
+    } else if (nq != 0 && !rows_fit && j == 0) {
+      rows_range(J, cstart, lo[1], hi[1], ra, rb);
+    }
+  } else {
+    auto row_of = [&](uint32_t i) { return (uint64_t)(lo[2] + i / n1) * J.nb[1] + lo[1] + i % n1; };
+    if (rows_fit) {
+      if ((uint32_t)j < nr) {
+        const uint64_t row = row_of((uint32_t)j);
+        key_range(J, cstart, kk, row, row * J.nb[0] + lo[0], row * J.nb[0] + hi[0], ra, rb);
+      }
+      if ((uint32_t)j + W < nr) {
+        const uint64_t row = row_of((uint32_t)j + W);
+        key_range(J, cstart, kk, row, row * J.nb[0] + lo[0], row * J.nb[0] + hi[0], ra2, rb2);
+      }
+    } else if (nq != 0 && j == 0) {
+      rows_range(J, cstart, (uint64_t)lo[2] * J.nb[1] + lo[1], (uint64_t)hi[2] * J.nb[1] + hi[1], ra, rb);
+    }
   }
 
   // moment cells (rows [jlo, jlo + nir) x cells [clo, clo + nc)): inside the
@@ -808,7 +834,7 @@ __device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const f
   // used when they hold at least 4 points per cell on average
   uint32_t m1 = 0, m2 = 0, jlo = 0, nir = 0, clo = 0, nc = 0;
   bool inr = false;
-  if (J.flags & kJobMoments) {
+  if (D == 2 && (J.flags & kJobMoments)) {
     const uint32_t cf = J.mcf;
     const float h0 = J.h[0], h1 = J.h[1];
     const float mg0 = (fabsf(bmin[0]) + fabsf(bmax[0]) + fabsf(J.mn[0]) + h0) * 1.0e-5f;
@@ -833,8 +859,9 @@ __device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const f
   const uint32_t pts = sub_sum<W>(inr ? m2 - m1 : 0u);
   const bool use_m = nir != 0 && pts >= 4u * nir * nc;
 
-  // range table: [ra, rb), or [ra, m1) and [m2, rb) for an interior row
-  uint32_t a_len = rb - ra, b_pos = rb, b_len = 0;
+  // range table: [ra, rb), or [ra, m1) and [m2, rb) for an interior row;
+  // 3-D: [ra, rb) and [ra2, rb2)
+  uint32_t a_len = rb - ra, b_pos = D == 3 ? ra2 : rb, b_len = rb2 - ra2;
   if (use_m && inr) {
     a_len = m1 - ra;
     b_pos = m2;
@@ -858,7 +885,7 @@ __device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const f
   // holds only points inside every query's support, so each query adds one
   // polynomial term per interior row instead of one per cell.
   const bool mrow = use_m && (uint32_t)j < nir;
-  if (wave_max_u32(mrow ? 1u : 0u) != 0) {
+  if (D == 2 && wave_max_u32(mrow ? 1u : 0u) != 0) {
     const float k0 = J.hinv[0] * J.hinv[0], k1 = J.hinv[1] * J.hinv[1];
     float M[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float mo0 = 0.f, mo1 = 0.f;
@@ -914,9 +941,9 @@ __device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const f
   // moves forward.  The lane keeps range kc's offset (sorted position less
   // flattened index) and the next range's first index in registers and
   // reads the table only when it moves on.
-  float sq[3][1];
+  float sq[3][1] = {{0.0f}, {0.0f}, {0.0f}};
 #pragma unroll
-  for (int e = 0; e < 2; ++e) sq[e][0] = exact ? xq[e] : (xq[e] - o[e]) * hinv[e];
+  for (int e = 0; e < D; ++e) sq[e][0] = exact ? xq[e] : (xq[e] - o[e]) * hinv[e];
   const uint32_t steps = wave_max_u32((total + kNC - 1) / kNC);
   const uint2* rt = L.rng + 2 * W * s;
   uint32_t kc = 0;
@@ -926,9 +953,10 @@ __device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const f
   // than 2^30 points)
   const char* c0 = reinterpret_cast<const char*>(src0);
   const char* c1 = reinterpret_cast<const char*>(src1);
-  auto load = [&](uint32_t step, float (&x)[G][2]) {
+  const char* c2 = reinterpret_cast<const char*>(src2);
+  auto load = [&](uint32_t step, float (&x)[kLd][D]) {
 #pragma unroll
-    for (int k = 0; k < G; ++k) {
+    for (int k = 0; k < kLd; ++k) {
       const uint32_t f = step * kNC + (uint32_t)(k * W + j);
       if (f < total) {
         while (f >= nx) {
@@ -939,29 +967,35 @@ __device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const f
         const uint32_t byte = (f + off) * 4u;
         x[k][0] = *reinterpret_cast<const float*>(c0 + byte);
         x[k][1] = *reinterpret_cast<const float*>(c1 + byte);
+        if constexpr (D == 3) x[k][2] = *reinterpret_cast<const float*>(c2 + byte);
       } else {
-        x[k][0] = kSentinel;
-        x[k][1] = kSentinel;
+#pragma unroll
+        for (int e = 0; e < D; ++e) x[k][e] = kSentinel;
       }
     }
   };
-  float nxt[G][2];
+  float nxt[kLd][D];
   load(0u, nxt);
-  float2* stage = reinterpret_cast<float2*>(L.gcand + s * kSubCandStride);
+  // the sub-tile's stage: 2-D candidates as float2, 3-D ones as float4
+  float4* stage4 = L.gcand + s * kSubCandStride;
+  float2* stage2 = reinterpret_cast<float2*>(stage4);
   auto run = [&](auto exact_tag) {
     constexpr bool kRawPairs = decltype(exact_tag)::value;
     for (uint32_t b = 0; b < steps; ++b) {
-      float c[G][2];
+      float c[kLd][D];
 #pragma unroll
-      for (int k = 0; k < G; ++k)
+      for (int k = 0; k < kLd; ++k)
 #pragma unroll
-        for (int e = 0; e < 2; ++e) c[k][e] = kRawPairs ? nxt[k][e] : (nxt[k][e] - o[e]) * hinv[e];
+        for (int e = 0; e < D; ++e) c[k][e] = kRawPairs ? nxt[k][e] : (nxt[k][e] - o[e]) * hinv[e];
       wave_sync();
 #pragma unroll
-      for (int k = 0; k < G; ++k) stage[k * W + j] = make_float2(c[k][0], c[k][1]);
+      for (int k = 0; k < kLd; ++k) {
+        if constexpr (D == 2) stage2[k * W + j] = make_float2(c[k][0], c[k][1]);
+        else stage4[k * W + j] = make_float4(c[k][0], c[k][1], c[k][2], 0.0f);
+      }
       wave_sync();
       if (b + 1 < steps) load(b + 1, nxt);
-      accumulate_chunk<2, 1, kRawPairs, false, kNC>(L.gcand + s * kSubCandStride, sq, hinv, S, C);
+      accumulate_chunk<D, 1, kRawPairs, false, kNC>(stage4, sq, hinv, S, C);
     }
   };
   if (exact) run(std::true_type{});
@@ -986,10 +1020,12 @@ __device__ void process_sub2(const DJob& J, const DTile& T, bool packed, const f
     st2 += (uint64_t)__builtin_amdgcn_readlane((uint32_t)mtt, g * W) |
            ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(mtt >> 32), g * W) << 32);
   }
-  ws.add(kStPairs + 1, sc);
+  ws.add(kStPairs + D - 1, sc);
   ws.add(kStSlots, (uint64_t)kWave * kNC * steps);
-  ws.add(kStMomPairs + 1, sm);
-  ws.add(kStMomTerms + 1, st2);
+  if constexpr (D == 2) {
+    ws.add(kStMomPairs + 1, sm);
+    ws.add(kStMomTerms + 1, st2);
+  }
   if (exact) ws.add(kStRaw, 1);
 }
 
@@ -1027,8 +1063,11 @@ __device__ __forceinline__ void run_tile(const DJob& J, const DTile& T, const fl
 }  // namespace
 
 // Persistent density kernel: every wave pulls tiles from the claim counters
-// until the tile list is exhausted.
-template <int Q, bool RAW, typename KeyT>
+// until the tile list is exhausted.  SUBD: the job dimension whose tiles the
+// one-query-per-lane variant runs as sub-tiles (2 or 3; tiles of the other
+// dimension run whole or packed), so that each variant holds the registers
+// of one sub-tile path only.
+template <int Q, bool RAW, typename KeyT, int SUBD>
 __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const DJob* __restrict__ jobs,
                                                     const DTile* __restrict__ tiles,
                                                     const uint32_t* __restrict__ n_tiles_p,
@@ -1075,27 +1114,40 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
         if (pk & 3u) {
           const bool four = (pk & 3u) == 3u;
           if (d == 2) {
-            // the variant with one query per lane runs packed 2-D tiles as
-            // sub-tiles (with moments); the others keep the lean pack path,
-            // so the sub-tile code does not add to their register pressure
-            if constexpr (Q == 1) {
-              if (four) process_sub2<4, KeyT>(J, T, true, scoord, keys, cstart, idx, mom2, dens, L, ws);
-              else process_sub2<2, KeyT>(J, T, true, scoord, keys, cstart, idx, mom2, dens, L, ws);
+            // the variant with one query per lane and 2-D sub-tiles runs
+            // packed 2-D tiles as sub-tiles (with moments); the others keep
+            // the lean pack path, so the sub-tile code does not add to their
+            // register pressure
+            if constexpr (Q == 1 && SUBD == 2) {
+              if (four) process_subt<2, 4, KeyT>(J, T, true, scoord, keys, cstart, idx, mom2, dens, L, ws);
+              else process_subt<2, 2, KeyT>(J, T, true, scoord, keys, cstart, idx, mom2, dens, L, ws);
             } else {
               ws.add(kStPairs + 1, four ? process_pack<2, 4, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, ws)
                                         : process_pack<2, 2, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, ws));
             }
           } else {
-            ws.add(kStPairs + 2, four ? process_pack<3, 4, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, ws)
-                                      : process_pack<3, 2, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, ws));
+            // packed 3-D tiles of jobs with sub-tiles, in the variant with
+            // 3-D sub-tiles: 32 candidates per sub-tile and step (the pack
+            // path: W)
+            bool done = false;
+            if constexpr (Q == 1 && SUBD == 3) {
+              if (__builtin_amdgcn_readfirstlane(J.flags) & (kJobSub16 | kJobSub32)) {
+                if (four) process_subt<3, 4, KeyT>(J, T, true, scoord, keys, cstart, idx, mom2, dens, L, ws);
+                else process_subt<3, 2, KeyT>(J, T, true, scoord, keys, cstart, idx, mom2, dens, L, ws);
+                done = true;
+              }
+            }
+            if (!done)
+              ws.add(kStPairs + 2, four ? process_pack<3, 4, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, ws)
+                                        : process_pack<3, 2, KeyT>(J, T, scoord, keys, cstart, idx, dens, L, ws));
           }
           continue;
         }
         if constexpr (Q == 1) {  // sub-tiles run with one query per lane
           const uint32_t jf = __builtin_amdgcn_readfirstlane(J.flags);
-          if (d == 2 && (jf & (kJobSub16 | kJobSub32))) {
-            if (jf & kJobSub16) process_sub2<4, KeyT>(J, T, false, scoord, keys, cstart, idx, mom2, dens, L, ws);
-            else process_sub2<2, KeyT>(J, T, false, scoord, keys, cstart, idx, mom2, dens, L, ws);
+          if (d == (uint32_t)SUBD && (jf & (kJobSub16 | kJobSub32))) {
+            if (jf & kJobSub16) process_subt<SUBD, 4, KeyT>(J, T, false, scoord, keys, cstart, idx, mom2, dens, L, ws);
+            else process_subt<SUBD, 2, KeyT>(J, T, false, scoord, keys, cstart, idx, mom2, dens, L, ws);
             continue;
           }
         }
@@ -1113,28 +1165,29 @@ __global__ __launch_bounds__(kBlock, ENT_DENSITY_MIN_WAVES) void k_density(const
 }
 
 // One kernel variant: Q queries per lane at most, RAW (plain path, keys
-// unused) or cell lists with KeyT keys.  Each variant is instantiated in one
-// of the kde_density_*.hip files (so the slow ones compile in parallel).
-template <int Q, bool RAW, typename KeyT>
+// unused) or cell lists with KeyT keys, sub-tiles of SUBD-dim jobs.  Each
+// variant is instantiated in one of the kde_density_*.hip files (so the slow
+// ones compile in parallel).
+template <int Q, bool RAW, typename KeyT, int SUBD>
 int density_occupancy_v(int* blocks_per_cu) {
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      blocks_per_cu, reinterpret_cast<const void*>(&k_density<Q, RAW, KeyT>), kBlock, 0);
+      blocks_per_cu, reinterpret_cast<const void*>(&k_density<Q, RAW, KeyT, SUBD>), kBlock, 0);
   return e == hipSuccess ? 0 : -1;
 }
 
-template <int Q, bool RAW, typename KeyT>
+template <int Q, bool RAW, typename KeyT, int SUBD>
 void launch_density_v(hipStream_t s, unsigned grid, const DJob* jobs, const DTile* tiles, const uint32_t* n_tiles,
                       uint32_t* counter, const float* X, const float* scoord, const void* keys,
                       const uint32_t* cstart, const uint32_t* idx, const float4* mom, const float* mom2,
                       double* dens, unsigned long long* pairs) {
-  hipLaunchKernelGGL((k_density<Q, RAW, KeyT>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles, counter, X,
+  hipLaunchKernelGGL((k_density<Q, RAW, KeyT, SUBD>), dim3(grid), dim3(kBlock), 0, s, jobs, tiles, n_tiles, counter, X,
                      scoord, (const KeyT*)keys, cstart, idx, mom, mom2, dens, pairs);
 }
 
 // explicit instantiation of one variant
-#define ENT_DENSITY_VARIANT(Q, RAW, KeyT)                                                                    \
-  template int density_occupancy_v<Q, RAW, KeyT>(int*);                                                     \
-  template void launch_density_v<Q, RAW, KeyT>(hipStream_t, unsigned, const DJob*, const DTile*,            \
+#define ENT_DENSITY_VARIANT(Q, RAW, KeyT, SUBD)                                                              \
+  template int density_occupancy_v<Q, RAW, KeyT, SUBD>(int*);                                               \
+  template void launch_density_v<Q, RAW, KeyT, SUBD>(hipStream_t, unsigned, const DJob*, const DTile*,            \
                                                const uint32_t*, uint32_t*, const float*, const float*,      \
                                                const void*, const uint32_t*, const uint32_t*, const float4*, \
                                                const float*, double*, unsigned long long*);

@@ -2,7 +2,7 @@
 #include "kde_density.hpp"
 
 namespace ent {
-ENT_DENSITY_VARIANT(2, true, uint32_t)
-ENT_DENSITY_VARIANT(2, false, uint32_t)
-ENT_DENSITY_VARIANT(2, false, uint64_t)
+ENT_DENSITY_VARIANT(2, true, uint32_t, 2)
+ENT_DENSITY_VARIANT(2, false, uint32_t, 2)
+ENT_DENSITY_VARIANT(2, false, uint64_t, 2)
 }  // namespace ent

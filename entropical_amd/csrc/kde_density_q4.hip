@@ -2,6 +2,6 @@
 #include "kde_density.hpp"
 
 namespace ent {
-ENT_DENSITY_VARIANT(4, true, uint32_t)
-ENT_DENSITY_VARIANT(4, false, uint32_t)
+ENT_DENSITY_VARIANT(4, true, uint32_t, 2)
+ENT_DENSITY_VARIANT(4, false, uint32_t, 2)
 }  // namespace ent

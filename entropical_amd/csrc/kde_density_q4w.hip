@@ -2,5 +2,5 @@
 #include "kde_density.hpp"
 
 namespace ent {
-ENT_DENSITY_VARIANT(4, false, uint64_t)
+ENT_DENSITY_VARIANT(4, false, uint64_t, 2)
 }  // namespace ent

@@ -35,7 +35,7 @@ constexpr uint32_t kJobMoments = 16u;  // 1-D or 2-D job with a moment table: th
                                        // or cells (2-D) that lie inside every query's
                                        // support are summed from their moments instead
                                        // of pair by pair
-constexpr uint32_t kJobSub16 = 32u;  // 2-D job whose tiles run as four sub-tiles of 16
+constexpr uint32_t kJobSub16 = 32u;  // 2-D or 3-D job whose tiles run as four sub-tiles of 16
                                      // queries, each with its own candidate stream
 constexpr uint32_t kJobSub32 = 64u;  // the same with two sub-tiles of 32 queries
 constexpr uint32_t kJobRowTable = 128u;  // cbase locates a row-start table (the first sorted

@@ -1037,42 +1037,46 @@ void launch_tile_pack(hipStream_t s, const DJob* jobs, DTile* tiles, const uint3
 }
 
 // Density kernel variants for q = 1, 2, 4 queries per lane, plain path or
-// cell lists with 32- or 64-bit keys (compiled in kde_density_*.hip).
-template <int Q>
+// cell lists with 32- or 64-bit keys, and for q = 1 the variant with 3-D
+// sub-tiles (compiled in kde_density_*.hip).
+template <int Q, int SUBD>
 static int occupancy_q(bool raw, bool wide, int* bpc) {
-  if (raw) return density_occupancy_v<Q, true, uint32_t>(bpc);
-  return wide ? density_occupancy_v<Q, false, uint64_t>(bpc) : density_occupancy_v<Q, false, uint32_t>(bpc);
+  if (raw) return density_occupancy_v<Q, true, uint32_t, 2>(bpc);
+  return wide ? density_occupancy_v<Q, false, uint64_t, SUBD>(bpc) : density_occupancy_v<Q, false, uint32_t, SUBD>(bpc);
 }
 
-int density_occupancy(int q, bool raw, bool wide, int* blocks_per_cu) {
+int density_occupancy(int q, bool raw, bool wide, bool sub3, int* blocks_per_cu) {
   switch (q) {
-    case 1: return occupancy_q<1>(raw, wide, blocks_per_cu);
-    case 2: return occupancy_q<2>(raw, wide, blocks_per_cu);
-    case 4: return occupancy_q<4>(raw, wide, blocks_per_cu);
+    case 1: return sub3 ? occupancy_q<1, 3>(raw, wide, blocks_per_cu) : occupancy_q<1, 2>(raw, wide, blocks_per_cu);
+    case 2: return occupancy_q<2, 2>(raw, wide, blocks_per_cu);
+    case 4: return occupancy_q<4, 2>(raw, wide, blocks_per_cu);
     default: return -1;
   }
 }
 
-int launch_density(hipStream_t s, int q, bool raw, bool wide, unsigned grid, const DJob* jobs,
+int launch_density(hipStream_t s, int q, bool raw, bool wide, bool sub3, unsigned grid, const DJob* jobs,
                    const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
                    const float* scoord, const void* keys, const uint32_t* cstart, const uint32_t* idx,
                    const float4* mom, const float* mom2, double* dens, unsigned long long* pairs) {
-#define ENT_LD(QQ)                                                                                              \
-  do {                                                                                                          \
-    if (raw)                                                                                                    \
-      launch_density_v<QQ, true, uint32_t>(s, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart, idx, \
-                                           mom, mom2, dens, pairs);                                             \
-    else if (wide)                                                                                              \
-      launch_density_v<QQ, false, uint64_t>(s, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart,   \
-                                            idx, mom, mom2, dens, pairs);                                       \
-    else                                                                                                        \
-      launch_density_v<QQ, false, uint32_t>(s, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart,   \
-                                            idx, mom, mom2, dens, pairs);                                       \
+#define ENT_LD(QQ, SD)                                                                                            \
+  do {                                                                                                            \
+    if (raw)                                                                                                      \
+      launch_density_v<QQ, true, uint32_t, 2>(s, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart,   \
+                                              idx, mom, mom2, dens, pairs);                                      \
+    else if (wide)                                                                                                \
+      launch_density_v<QQ, false, uint64_t, SD>(s, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart, \
+                                                idx, mom, mom2, dens, pairs);                                    \
+    else                                                                                                          \
+      launch_density_v<QQ, false, uint32_t, SD>(s, grid, jobs, tiles, n_tiles, counter, X, scoord, keys, cstart, \
+                                                idx, mom, mom2, dens, pairs);                                    \
   } while (0)
   switch (q) {
-    case 1: ENT_LD(1); break;
-    case 2: ENT_LD(2); break;
-    case 4: ENT_LD(4); break;
+    case 1:
+      if (sub3) ENT_LD(1, 3);
+      else ENT_LD(1, 2);
+      break;
+    case 2: ENT_LD(2, 2); break;
+    case 4: ENT_LD(4, 2); break;
     default: return -1;
   }
 #undef ENT_LD

@@ -55,16 +55,18 @@ constexpr uint32_t kClaimParts = ENT_CLAIM_PARTS;
 constexpr uint32_t kClaimStride = 1024;
 constexpr uint32_t kCounterWords = kClaimPartsMax * kClaimStride;
 static_assert(kClaimParts >= 1 && kClaimParts <= kClaimPartsMax, "claim partitions");
-int density_occupancy(int q, bool raw, bool wide, int* blocks_per_cu);
+// sub3: the one-query-per-lane variant that runs 3-D jobs' tiles as
+// sub-tiles (2-D ones whole), for batches with 3-D jobs and sub-tiles
+int density_occupancy(int q, bool raw, bool wide, bool sub3, int* blocks_per_cu);
 // One density kernel variant (instantiated in the kde_density_*.hip files).
-template <int Q, bool RAW, typename KeyT>
+template <int Q, bool RAW, typename KeyT, int SUBD>
 int density_occupancy_v(int* blocks_per_cu);
-template <int Q, bool RAW, typename KeyT>
+template <int Q, bool RAW, typename KeyT, int SUBD>
 void launch_density_v(hipStream_t s, unsigned grid, const DJob* jobs, const DTile* tiles, const uint32_t* n_tiles,
                       uint32_t* counter, const float* X, const float* scoord, const void* keys,
                       const uint32_t* cstart, const uint32_t* idx, const float4* mom, const float* mom2,
                       double* dens, unsigned long long* pairs);
-int launch_density(hipStream_t s, int q, bool raw, bool wide, unsigned grid, const DJob* jobs,
+int launch_density(hipStream_t s, int q, bool raw, bool wide, bool sub3, unsigned grid, const DJob* jobs,
                    const DTile* tiles, const uint32_t* n_tiles, uint32_t* counter, const float* X,
                    const float* scoord, const void* keys, const uint32_t* cstart, const uint32_t* idx,
                    const float4* mom, const float* mom2, double* dens, unsigned long long* pairs);

@@ -73,9 +73,10 @@ int ent_ctx_destroy(ent_ctx* ctx);
 
 /* Engine tuning: maximum queries per lane (1,2,4; smaller tiles use fewer), dim-0 boxes per bandwidth,
  * row-dim boxes per bandwidth, tile extent bound in bandwidths, and the
- * sub-tile size of 2-D jobs (0: whole tiles; 16 or 32, with q = 1: tiles of
- * up to 64 queries run as sub-tiles of that many queries, each with its own
- * candidate stream).  Replaces the reference's compile-time BSIZE
+ * sub-tile size (0: whole tiles; 16 or 32, with q = 1: tiles of up to 64
+ * queries of 2-D jobs -- or, in batches with 3-D jobs and mr <= 2, of 3-D
+ * jobs -- run as sub-tiles of that many queries, each with its own candidate
+ * stream).  Replaces the reference's compile-time BSIZE
  * (densities_cuda.cu:14). */
 int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h, int sub);
 
@@ -282,8 +283,10 @@ int ent_hestimate_dev(ent_ctx* ctx, const float* X_dev, size_t n_rows, size_t n_
 /* Tile-shape autotuner (no reference counterpart; the reference fixes its
  * tile at compile time, densities_cuda.cu:14).  Times every setting of the
  * grid q in {1,2,4}, m0 in {8,16,32}, mr in {1,2}, group_h in {4,16,64} with
- * whole 2-D tiles, and for batches with 2-D jobs also q = 1, m0 in {32,64},
- * mr in {2,3,4}, group_h = 64 with 2-D sub-tiles of 16, on a
+ * whole tiles, for batches with 2-D jobs also q = 1, m0 in {32,64},
+ * mr in {2,3,4}, group_h = 64 with sub-tiles of 16, and for batches with
+ * 3-D jobs q = 1, m0 in {16,32}, mr in {1,2}, group_h = 64 with sub-tiles of
+ * 16 (3-D jobs; their 2-D jobs then run whole tiles), on a
  * subsample of the job set (the four fastest re-timed on the whole set),
  * keeps the fastest on the context (and returns it in tuning5 = (q, m0, mr,
  * group_h, sub) when non-NULL),

@@ -14,15 +14,19 @@ from entropical_amd import (Context, DeviceArray, EntropicalError, batch_autotun
 pytestmark = pytest.mark.gpu
 
 GRID_Q, GRID_M0, GRID_MR, GRID_GH = (1, 2, 4), (8, 16, 32), (1, 2), (4, 16, 64)
-# 2-D sub-tiles of 16 (batches with 2-D jobs)
+# sub-tiles of 16: q = 1, group_h = 64, and (m0, mr) from the 2-D grid
+# (batches with 2-D jobs) or the 3-D one (batches with 3-D jobs)
 SUB_Q, SUB_M0, SUB_MR = (1,), (32, 64), (2, 3, 4)
+SUB3_M0, SUB3_MR = (16, 32), (1, 2)
 
 
-def in_grid(t, has_2d=True):
+def in_grid(t, has_2d=True, has_3d=False):
     q, m0, mr, gh, sub = t
     if sub == 0:
         return q in GRID_Q and m0 in GRID_M0 and mr in GRID_MR and gh in GRID_GH
-    return has_2d and sub == 16 and q in SUB_Q and m0 in SUB_M0 and mr in SUB_MR and gh == 64
+    if sub != 16 or q not in SUB_Q or gh != 64:
+        return False
+    return (has_2d and m0 in SUB_M0 and mr in SUB_MR) or (has_3d and m0 in SUB3_M0 and mr in SUB3_MR)
 
 
 def rel_err(a, b):
@@ -82,12 +86,12 @@ def test_batch_autotune_signature_and_cache_hit(ctx, orc, monkeypatch, tmp_path)
                     {"n_dim": 2, "n": n, "off": [0, n], "h": [0.15 * scale, 0.2 * scale]},
                     {"n_dim": 3, "n": n, "off": [0, n, 2 * n], "h": [0.3 * scale, 0.3 * scale, 0.3 * scale]}]
         t1, ms1 = batch_autotune(ctx, dx.ptr, jobs(1.0), return_ms=True)
-        assert in_grid(t1) and ms1 > 0
+        assert in_grid(t1, has_3d=True) and ms1 > 0
         t2, ms2 = batch_autotune(ctx, dx.ptr, jobs(1.0), return_ms=True)
         assert t2 == t1 and ms2 == -1.0  # cache hit
         # the same jobs at 4x the bandwidths: another signature, searched again
         t3, ms3 = batch_autotune(ctx, dx.ptr, jobs(4.0), return_ms=True)
-        assert in_grid(t3) and ms3 > 0
+        assert in_grid(t3, has_3d=True) and ms3 > 0
         # densities at the tuning left on the context
         got = combined_densities(x, n, [0, 1, 2], [1.2, 1.2, 1.2], ctx=ctx)
         assert rel_err(got, orc.density_fp64_window(x, n, [0, 1, 2], [1.2, 1.2, 1.2])) < 1e-5

@@ -347,6 +347,78 @@ def test_sub_tiles_packed_and_sparse(ctx, orc, monkeypatch):
         ctx.set_tuning()
 
 
+@pytest.mark.parametrize("sub,mr", [(16, 1), (32, 1), (16, 2)])
+def test_sub_tiles_3d(ctx, orc, sub, mr):
+    """3-D tiles run as sub-tiles of 16 or 32 queries (the one-query-per-lane
+    kernel variant with 3-D sub-tiles), each with its own bounding box and
+    candidate rows (two range entries per lane: up to 2W rows).  The
+    densities match FP64 and the whole-tile path's within the tolerance; the
+    sub-tiles' candidate windows are narrower along dim 0, so fewer pairs are
+    evaluated than with whole tiles of the same shape."""
+    from entropical_amd import combined_densities
+
+    r = np.random.default_rng(161 + sub + mr)
+    n = 60000
+    # (x_t, x_t+1, y_t) of an AR(1) series x and a noise series y, as a
+    # transfer-entropy job lays them out
+    e = r.standard_normal(n + 1)
+    xs = np.empty(n + 1)
+    xs[0] = e[0]
+    for t in range(1, n + 1):
+        xs[t] = 0.8 * xs[t - 1] + 0.6 * e[t]
+    x = np.stack([xs[:n], xs[1:], r.standard_normal(n)]).astype(np.float32)
+    h = [0.25, 0.25, 0.3]
+    ref = orc.density_fp64_window(x, n, [0, 1, 2], h)
+    got, st = {}, {}
+    try:
+        for s_ in (0, sub):
+            ctx.set_tuning(1, 32, mr, 64, s_)
+            got[s_] = combined_densities(x, n, [0, 1, 2], h, ctx=ctx)
+            st[s_] = ctx.last_stats()
+            assert rel_err(got[s_], ref) < RTOL, s_
+    finally:
+        ctx.set_tuning()
+    ev = {k: v["pairs_by_d"][2] for k, v in st.items()}
+    assert 0 < ev[sub] < 0.9 * ev[0], ev
+
+
+def test_sub_tiles_3d_batch_packed_and_sparse(ctx, orc, monkeypatch):
+    """A batch of 1-D, 2-D and 3-D jobs at a tuning with sub-tiles: the 3-D
+    jobs run as sub-tiles, the 2-D ones as whole tiles (the kernel variant
+    with 3-D sub-tiles), the 1-D ones as usual.  Heavy-tailed 3-D data give
+    short tiles, packed (ENT_TILE_PACK) into the sub-tile path with the
+    offsets of the pack, and tiles wider than 32 bandwidths (raw-coordinate
+    pair terms).  All match FP64, with packing on and off."""
+    from entropical_amd import DeviceArray, batch_densities_dev
+
+    r = np.random.default_rng(167)
+    n = 20000
+    x = np.concatenate([r.standard_normal((3, n)), r.standard_t(1.5, size=(3, n)) * 2.0]).astype(np.float32)
+    jobs = [{"n_dim": 1, "n": n, "n_rows": n, "off": [0], "h": [0.1], "out_off": 0},
+            {"n_dim": 2, "n": n, "n_rows": n, "off": [0, n], "h": [0.2, 0.25], "out_off": n},
+            {"n_dim": 3, "n": n, "n_rows": n, "off": [0, n, 2 * n], "h": [0.3, 0.3, 0.35], "out_off": 2 * n},
+            {"n_dim": 3, "n": n, "n_rows": n, "off": [3 * n, 4 * n, 5 * n], "h": [0.05, 0.4, 0.4],
+             "out_off": 3 * n}]
+    refs = [orc.density_fp64_window(x, n, [o // n for o in j["off"]], j["h"]) for j in jobs]
+    dx = DeviceArray(ctx, x)
+    out = ctx.alloc(8 * len(jobs) * n)
+    try:
+        for sub in (16, 32):
+            ctx.set_tuning(1, 32, 1, 64, sub)
+            for pack in ("0", "1"):
+                monkeypatch.setenv("ENT_TILE_PACK", pack)
+                batch_densities_dev(ctx, dx.ptr, jobs, out)
+                got = np.empty(len(jobs) * n)
+                ctx.d2h(got, out)
+                for k, ref in enumerate(refs):
+                    assert rel_err(got[k * n:(k + 1) * n], ref) < RTOL, (sub, pack, k)
+            assert ctx.last_stats()["raw_tiles"] > 0
+    finally:
+        ctx.free(out)
+        dx.free()
+        ctx.set_tuning()
+
+
 def test_moment_tables_over_cap_fall_back(ctx, orc, monkeypatch):
     """Moment tables that do not fit (here: a 1-byte cap) leave every pair to
     the pair loop: same densities within tolerance, no moment pairs, and the
