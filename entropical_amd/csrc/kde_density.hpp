@@ -214,6 +214,44 @@ __device__ __forceinline__ void accumulate_chunk(const float4* c4, const float (
   for (int g = 0; g < kReads; ++g) consume(c4[g], g);
 }
 
+// 3-D candidates staged as structure of arrays (x of NC candidates, then y,
+// then z; NC a multiple of 32) against one query per lane: one float4 read
+// per coordinate serves four candidates (three reads per four candidates
+// instead of four), compensated merge every 32.
+template <bool RAW, int NC>
+__device__ __forceinline__ void accumulate_soa3(const float4* st, const float (&sq)[3][1], const float (&hinv)[3],
+                                                float& S, float& C) {
+  static_assert(NC % 32 == 0, "merge every 32 candidates");
+  float part = 0.0f;
+#pragma unroll
+  for (int g = 0; g < NC / 4; ++g) {
+    const float4 v[3] = {st[g], st[NC / 4 + g], st[NC / 2 + g]};
+    float cs[3][4];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      cs[e][0] = v[e].x;
+      cs[e][1] = v[e].y;
+      cs[e][2] = v[e].z;
+      cs[e][3] = v[e].w;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      float w[3];
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        float uu = sq[e][0] - cs[e][c];
+        if constexpr (RAW) uu = uu * hinv[e];
+        w[e] = clamp01(__builtin_fmaf(-uu, uu, 1.0f));
+      }
+      part = __builtin_fmaf(w[0] * w[1], w[2], part);
+    }
+    if ((g + 1) % 8 == 0) {
+      kahan_add(S, C, part);
+      part = 0.0f;
+    }
+  }
+}
+
 // Moment boxes [b0, b0 + nbx) of a 1-D tile added into the lanes' queries:
 // 64 boxes at a time, lane k stages box k's origin and moments in LDS, then
 // every lane adds c + d (B - A d), d = xq - r, per box (padding boxes are
@@ -957,7 +995,9 @@ __device__ void process_subt(const DJob& J, const DTile& T, bool packed, const f
   auto load = [&](uint32_t step, float (&x)[kLd][D]) {
 #pragma unroll
     for (int k = 0; k < kLd; ++k) {
-      const uint32_t f = step * kNC + (uint32_t)(k * W + j);
+      // 2-D: lane j loads candidates k W + j of the step; 3-D: kLd
+      // consecutive ones, kLd j + k (one vector store per coordinate)
+      const uint32_t f = step * kNC + (uint32_t)(D == 3 ? kLd * j + k : k * W + j);
       if (f < total) {
         while (f >= nx) {
           ++kc;
@@ -991,11 +1031,22 @@ __device__ void process_subt(const DJob& J, const DTile& T, bool packed, const f
 #pragma unroll
       for (int k = 0; k < kLd; ++k) {
         if constexpr (D == 2) stage2[k * W + j] = make_float2(c[k][0], c[k][1]);
-        else stage4[k * W + j] = make_float4(c[k][0], c[k][1], c[k][2], 0.0f);
+      }
+      if constexpr (D == 3) {
+        // structure of arrays: x, y, z of the step's kNC candidates
+        float* st3 = reinterpret_cast<float*>(stage4);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          if constexpr (kLd == 2)
+            reinterpret_cast<float2*>(st3 + e * kNC)[j] = make_float2(c[0][e], c[1][e]);
+          else
+            st3[e * kNC + j] = c[0][e];
+        }
       }
       wave_sync();
       if (b + 1 < steps) load(b + 1, nxt);
-      accumulate_chunk<D, 1, kRawPairs, false, kNC>(stage4, sq, hinv, S, C);
+      if constexpr (D == 3) accumulate_soa3<kRawPairs, kNC>(stage4, sq, hinv, S[0], C[0]);
+      else accumulate_chunk<D, 1, kRawPairs, false, kNC>(stage4, sq, hinv, S, C);
     }
   };
   if (exact) run(std::true_type{});
