@@ -469,18 +469,52 @@ __global__ __launch_bounds__(kBlock) void k_scan_blocks(const uint32_t* in, uint
   if (threadIdx.x == kBlock - 1) block_sums[blockIdx.x] = s[kBlock - 1];
 }
 
+// Adds each tile's offset (the scanned tile sums, whose total follows them
+// at block_offs[nb]); block 0 also writes the total to out[m].
 __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* out, uint64_t m,
                                                      const uint32_t* block_offs) {
   const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
   const uint32_t add = block_offs[blockIdx.x];
   for (int i = threadIdx.x; i < kScanTile; i += kBlock)
     if (base + i < m) out[base + i] += add;
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[m] = block_offs[gridDim.x];
 }
 
-// Grand total (already reduced into *total) copied to out[m].  The input may
-// have been overwritten by an in-place scan, so the total never comes from it.
-__global__ void k_scan_total(const uint32_t* total, uint32_t* out, uint64_t m) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) out[m] = *total;
+// Exclusive scan of up to kScanSmallTiles tiles by one workgroup, tile after
+// tile with a running carry (in place allowed: each thread reads its items
+// before it writes them); the total goes to out[m].  One launch where the
+// tiled scan takes three or more.
+constexpr int kScanSmallTiles = 16;
+__global__ __launch_bounds__(kBlock) void k_scan_small(const uint32_t* in, uint32_t* out, uint64_t m) {
+  __shared__ uint32_t s[kBlock];
+  uint32_t carry = 0;
+  for (uint64_t t0 = 0; t0 < m; t0 += kScanTile) {
+    const uint64_t base = t0 + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      v[i] = (base + i < m) ? in[base + i] : 0u;
+      tsum += v[i];
+    }
+    s[threadIdx.x] = tsum;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {
+      uint32_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0u;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    uint32_t run = carry + s[threadIdx.x] - tsum;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      if (base + i < m) out[base + i] = run;
+      run += v[i];
+    }
+    carry += s[kBlock - 1];
+    __syncthreads();  // s is rewritten by the next tile
+  }
+  if (threadIdx.x == 0) out[m] = carry;
 }
 
 // ------------------------------------------------------- sorted gather
@@ -993,16 +1027,16 @@ void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uin
                            uint32_t* scratch) {
   if (m == 0) return;
   uint64_t nb = (m + kScanTile - 1) / kScanTile;
+  if (nb <= (uint64_t)kScanSmallTiles) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kBlock), 0, s, in, out, m);
+    return;
+  }
   uint32_t* sums = scratch;
   hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(kBlock), 0, s, in, out, m, sums);
-  if (nb > 1) {
-    // scans the block sums in place; their total lands in sums[nb]
-    launch_exclusive_scan(s, sums, sums, nb, scratch + nb + 1);
-    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kBlock), 0, s, out, m, sums);
-    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, s, sums + nb, out, m);
-  } else {
-    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, s, sums, out, m);
-  }
+  // scans the block sums in place; their total lands in sums[nb], which
+  // k_scan_add copies to out[m]
+  launch_exclusive_scan(s, sums, sums, nb, scratch + nb + 1);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kBlock), 0, s, out, m, sums);
 }
 
 void launch_gather_sorted(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,

@@ -114,8 +114,10 @@ def test_entropy1d_100000(ctx, orc, threads):
     assert abs(ref_H - H_true) < 0.02, (ref_H, H_true)
 
 
-def test_te_matrix_32x100000(ctx, orc, threads):
-    """The 4-lag TE set at 32 x 100k.  The same batch run through
+@pytest.mark.parametrize("tuning", [(2, 16, 2, 16, 0), (1, 32, 1, 64, 16)])
+def test_te_matrix_32x100000(ctx, orc, threads, tuning):
+    """The 4-lag TE set at 32 x 100k, with whole tiles and with 3-D
+    sub-tiles of 16 (the autotuned shape).  The same batch run through
     ent_batch_densities_dev gives a driven pair's 3-D densities
     p(x_t, x_t+tau, y_t) at every lag (and the reverse direction's at the
     largest lag) within 1e-5 of FP64 per point; the TE matrix entries of
@@ -136,8 +138,8 @@ def test_te_matrix_32x100000(ctx, orc, threads):
     a, b = min(i, j), max(i, j)
     dev = {}
     try:
+        ctx.set_tuning(*tuning)
         te = te_matrix(ctx, dx, h, taus)
-        ctx.set_tuning(2, 16, 2, 16)
         batch_densities_dev(ctx, dx.ptr, jobs, out)
         for t, tau in enumerate(taus):
             g = np.empty(n - tau)
