@@ -703,16 +703,24 @@ __global__ __launch_bounds__(kBlock) void k_cell_moments2(const DJob* __restrict
         }
       }
     }
+    // the block's cells leave through LDS (the point stage, free now) so
+    // that its 9 x kBlock floats are stored contiguously, one float per lane
+    // and instruction, instead of nine stores 36 bytes apart per lane
+    __syncthreads();
+    float* ostage = reinterpret_cast<float*>(stage);
     if (valid) {
       const double k0 = (double)J.hinv[0] * (double)J.hinv[0];
       const double k1 = (double)J.hinv[1] * (double)J.hinv[1];
       const double s0[3] = {1.0, 2.0 * k0, -k0}, s1[3] = {1.0, 2.0 * k1, -k1};
-      float* out = mom + (J.mbase + cell) * 9;
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) out[3 * i + k] = (float)(m[3 * i + k] * s0[i] * s1[k]);
+        for (int k = 0; k < 3; ++k) ostage[9 * threadIdx.x + 3 * i + k] = (float)(m[3 * i + k] * s0[i] * s1[k]);
     }
+    __syncthreads();
+    float* out = mom + (J.mbase + cell0) * 9;
+    const uint32_t nf = 9 * (last + 1);
+    for (uint32_t i = threadIdx.x; i < nf; i += kBlock) out[i] = ostage[i];
   }
 }
 
