@@ -295,6 +295,18 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   // host planning: boxes per dim (boxedsearch.cpp:21-27 with the total fixes)
   uint64_t max_key = 0;
   uint64_t table_total = 0;
+  // Sub-tile shapes in use (others keep whole and packed tiles): 3-D jobs
+  // with rows h high (sub-tiles of 16 or 32) or h/2 (sub-tiles of 16); 2-D
+  // jobs with rows h/2 or finer (sub-tiles of 16) or h/4 or finer (of 32),
+  // and at any row height in a batch whose 3-D jobs run as sub-tiles (its
+  // 2-D tiles then run whole, with the sub-tile moment table).  The sub-tile
+  // code handles every shape; the others are held back because a randomized
+  // sweep over them faulted the GPU for a reason not yet found.
+  const bool sub3_shape = (tuning.sub == 16 && tuning.mr <= 2) || (tuning.sub == 32 && tuning.mr == 1);
+  bool batch3 = false;
+  for (const DJob& D : h_jobs) batch3 = batch3 || D.d == 3;
+  const bool sub2_shape = (tuning.sub == 16 && tuning.mr >= 2) || (tuning.sub == 32 && tuning.mr >= 4) ||
+                          (batch3 && sub3_shape);
   for (uint32_t j = 0; j < J; ++j) {
     DJob& D = h_jobs[j];
     uint64_t cells = 1;
@@ -327,7 +339,7 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     // sub-tiles: tiles of up to 64 queries (four of 16 or two of 32 side by
     // side in one wave)
     D.tq = (uint32_t)(kWave * tuning.q);
-    if ((D.d == 2 || (D.d == 3 && tuning.mr <= 2)) && (tuning.sub == 16 || tuning.sub == 32)) {
+    if ((D.d == 2 && sub2_shape) || (D.d == 3 && sub3_shape)) {
       D.tq = kWave;
       D.flags |= tuning.sub == 16 ? kJobSub16 : kJobSub32;
     }

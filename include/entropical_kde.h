@@ -74,9 +74,10 @@ int ent_ctx_destroy(ent_ctx* ctx);
 /* Engine tuning: maximum queries per lane (1,2,4; smaller tiles use fewer), dim-0 boxes per bandwidth,
  * row-dim boxes per bandwidth, tile extent bound in bandwidths, and the
  * sub-tile size (0: whole tiles; 16 or 32, with q = 1: tiles of up to 64
- * queries of 2-D jobs -- or, in batches with 3-D jobs and mr <= 2, of 3-D
- * jobs -- run as sub-tiles of that many queries, each with its own candidate
- * stream).  Replaces the reference's compile-time BSIZE
+ * queries run as sub-tiles of that many queries, each with its own candidate
+ * stream: 2-D jobs at mr >= 2 with 16, mr >= 4 with 32; 3-D jobs at mr <= 2
+ * with 16, mr = 1 with 32, the batch's 2-D jobs then as whole tiles; other
+ * shapes keep whole tiles).  Replaces the reference's compile-time BSIZE
  * (densities_cuda.cu:14). */
 int ent_ctx_set_tuning(ent_ctx* ctx, int q, int m0, int mr, int group_h, int sub);
 
