@@ -82,7 +82,12 @@ int reduce_only(ent_ctx* ctx, const std::vector<ent::RedItem>& items, std::vecto
   const uint32_t NI = (uint32_t)items.size();
   uint64_t maxn = 1;
   for (const auto& it : items) maxn = std::max<uint64_t>(maxn, it.n);
-  uint32_t bpi = (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, maxn / 16384));
+  // blocks per item: one per 16384 terms, and for batches of few items up to
+  // 1024 blocks in all (at least 1024 terms each), so a single long item
+  // does not run on a handful of workgroups; at most 64
+  uint64_t want = std::max<uint64_t>(maxn / 16384, (1024 + NI - 1) / std::max<uint32_t>(NI, 1));
+  want = std::min<uint64_t>(want, std::max<uint64_t>(1, maxn / 1024));
+  uint32_t bpi = (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, want));
   size_t items_bytes = (sizeof(ent::RedItem) * NI + 255) / 256 * 256;
   size_t part_bytes = sizeof(double) * (size_t)NI * bpi;
   DRV_TRY(E.ensure(ctx->red, items_bytes + part_bytes + sizeof(double) * NI));

@@ -502,8 +502,9 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   bool staged = true;
   if (const char* m = std::getenv("ENT_SCATTER"))
     if (!std::strcmp(m, "direct")) staged = false;
-  // packing of small tiles (on unless ENT_TILE_PACK=0)
-  bool pack = true;
+  // packing of small tiles (on unless ENT_TILE_PACK=0; only 2-D and 3-D
+  // jobs pack, so a batch of 1-D jobs skips the pass)
+  bool pack = std::any_of(h_jobs.begin(), h_jobs.end(), [](const DJob& D) { return D.d >= 2; });
   if (const char* m = std::getenv("ENT_TILE_PACK"))
     if (!std::strcmp(m, "0")) pack = false;
   const uint32_t NB = (uint32_t)h_bjobs.size();

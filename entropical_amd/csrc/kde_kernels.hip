@@ -441,9 +441,28 @@ __global__ __launch_bounds__(kBlock) void k_major_keys(const DJob* __restrict__ 
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kBlock * kScanItems;
 
+// Exclusive prefix of each thread's v over the workgroup (wave scans, then
+// the kBlock / kWave wave totals through LDS: one barrier) and the
+// workgroup's total.  The caller synchronises before wsum is written again.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t& total) {
+  const uint32_t incl = wave_incl_scan(v);
+  const uint32_t w = threadIdx.x / kWave;
+  if (lane_id() == kWave - 1) wsum[w] = incl;
+  __syncthreads();
+  uint32_t off = 0;
+  total = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < (uint32_t)(kBlock / kWave); ++i) {
+    const uint32_t t = wsum[i];
+    off += i < w ? t : 0u;
+    total += t;
+  }
+  return off + incl - v;
+}
+
 __global__ __launch_bounds__(kBlock) void k_scan_blocks(const uint32_t* in, uint32_t* out,
                                                         uint64_t m, uint32_t* block_sums) {
-  __shared__ uint32_t s[kBlock];
+  __shared__ uint32_t wsum[kBlock / kWave];
   const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t tsum = 0;
@@ -452,21 +471,14 @@ __global__ __launch_bounds__(kBlock) void k_scan_blocks(const uint32_t* in, uint
     v[i] = (base + i < m) ? in[base + i] : 0u;
     tsum += v[i];
   }
-  s[threadIdx.x] = tsum;
-  __syncthreads();
-  for (int o = 1; o < kBlock; o <<= 1) {
-    uint32_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0u;
-    __syncthreads();
-    s[threadIdx.x] += t;
-    __syncthreads();
-  }
-  uint32_t run = s[threadIdx.x] - tsum;
+  uint32_t total;
+  uint32_t run = block_excl_scan(tsum, wsum, total);
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
     if (base + i < m) out[base + i] = run;
     run += v[i];
   }
-  if (threadIdx.x == kBlock - 1) block_sums[blockIdx.x] = s[kBlock - 1];
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
 }
 
 // Adds each tile's offset (the scanned tile sums, whose total follows them
@@ -486,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* out, uint64_t m,
 // tiled scan takes three or more.
 constexpr int kScanSmallTiles = 16;
 __global__ __launch_bounds__(kBlock) void k_scan_small(const uint32_t* in, uint32_t* out, uint64_t m) {
-  __shared__ uint32_t s[kBlock];
+  __shared__ uint32_t wsum[kBlock / kWave];
   uint32_t carry = 0;
   for (uint64_t t0 = 0; t0 < m; t0 += kScanTile) {
     const uint64_t base = t0 + (uint64_t)threadIdx.x * kScanItems;
@@ -497,22 +509,15 @@ __global__ __launch_bounds__(kBlock) void k_scan_small(const uint32_t* in, uint3
       v[i] = (base + i < m) ? in[base + i] : 0u;
       tsum += v[i];
     }
-    s[threadIdx.x] = tsum;
-    __syncthreads();
-    for (int o = 1; o < kBlock; o <<= 1) {
-      uint32_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0u;
-      __syncthreads();
-      s[threadIdx.x] += t;
-      __syncthreads();
-    }
-    uint32_t run = carry + s[threadIdx.x] - tsum;
+    uint32_t total;
+    uint32_t run = carry + block_excl_scan(tsum, wsum, total);
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
       if (base + i < m) out[base + i] = run;
       run += v[i];
     }
-    carry += s[kBlock - 1];
-    __syncthreads();  // s is rewritten by the next tile
+    carry += total;
+    __syncthreads();  // wsum is rewritten by the next tile
   }
   if (threadIdx.x == 0) out[m] = carry;
 }
