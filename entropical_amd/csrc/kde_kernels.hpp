@@ -75,7 +75,9 @@ int launch_density(hipStream_t s, int q, bool raw, bool wide, bool sub3, unsigne
 // waves (sets the grid, which strides over them).
 void launch_box_moments(hipStream_t s, const DJob* jobs, const uint32_t* mlist, uint32_t max_waves,
                         const float* scoord, const uint32_t* start, float4* mom);
-constexpr uint32_t kMomentBoxesPerWave = 16;
+// Boxes per wave of k_box_moments (a wave sums its boxes one after another;
+// few per wave keep a single job's table from waiting on a few long waves).
+constexpr uint32_t kMomentBoxesPerWave = 4;
 // Moment table of the 2-D kJobMoments jobs listed in mlist ([0] = M, [1..M+1]
 // = first block per job, kBlock cells per block, then the M job indices);
 // max_blocks bounds the list's blocks.
