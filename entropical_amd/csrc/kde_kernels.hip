@@ -210,31 +210,59 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
   }
 }
 
-// LDS-staged stable scatter, one wavefront per block and chunk.  Phase 1
-// ranks the chunk's elements by digit (stable, as above) into LDS, so the
-// chunk sits there sorted by digit; phase 2 streams it out in that order:
-// consecutive lanes then write consecutive global slots within each digit's
-// run, instead of 64 unrelated addresses per store.  The chunk's per-digit
-// counts come from the scanned offsets: offs[i + 1] - offs[i] (the scan is
-// contiguous over all jobs and ends with the grand total at offs[m]).
+// Exclusive prefix of each thread's v over the workgroup (wave scans, then
+// the kBlock / kWave wave totals through LDS: one barrier) and the
+// workgroup's total.  The caller synchronises before wsum is written again.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t& total) {
+  const uint32_t incl = wave_incl_scan(v);
+  const uint32_t w = threadIdx.x / kWave;
+  if (lane_id() == kWave - 1) wsum[w] = incl;
+  __syncthreads();
+  uint32_t off = 0;
+  total = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < (uint32_t)(kBlock / kWave); ++i) {
+    const uint32_t t = wsum[i];
+    off += i < w ? t : 0u;
+    total += t;
+  }
+  return off + incl - v;
+}
+
+// LDS-staged stable scatter, one workgroup of kBlock threads per chunk.
+// Phase 1 ranks the chunk's elements by digit into LDS, so the chunk sits
+// there sorted by digit: wave w ranks the chunk's rounds [w R, w R + R) of
+// 64 elements (R = kChunkItems / waves; ballots give each element its rank
+// among the round's equal digits, a per-wave running count per digit its
+// rank within the wave), and each wave's run of a digit starts after those
+// of the waves before it, so equal digits keep the point order (stable).
+// Phase 2 streams the chunk out in that order: consecutive threads write
+// consecutive global slots within each digit's run, instead of unrelated
+// addresses per store.  The chunk's per-digit counts come from the scanned
+// offsets: offs[i + 1] - offs[i] (the scan is contiguous over all jobs and
+// ends with the grand total at offs[m]).
 // With GATHER (last pass) the full cell key is recomputed from the gathered
 // coordinates, so a pass over a partial key (the two-stage sort) also ends
 // with full keys.
+constexpr int kScatterWaves = kBlock / kWave;
+constexpr int kScatterRounds = kChunkItems / kScatterWaves;
+static_assert(kChunkItems % kScatterWaves == 0, "rounds split evenly over the waves");
+static_assert(kRadixBins == kBlock, "one digit per thread");
 // Output phase of a gathering scatter pass for jobs of dimension D: staged
-// points leave in digit order, kB per lane at a time (positions from LDS,
-// then the coordinate gathers, then the stores).
+// points leave in digit order, kB per thread at a time (positions from LDS,
+// then the coordinate gathers, then the stores); thread t of T.
 template <int D, typename KeyT>
 __device__ __forceinline__ void scatter_gather_out(const DJob& J, const KeyT* skey, const uint32_t* sid,
                                                    const uint32_t* delta, uint32_t nv, int shift,
                                                    const float* __restrict__ X, KeyT* __restrict__ keys_out,
                                                    uint32_t* __restrict__ idx_out, float* __restrict__ scoord,
-                                                   int l) {
-  constexpr int kB = 8;
-  for (uint32_t j0 = 0; j0 < nv; j0 += kB * kWave) {
+                                                   uint32_t t, uint32_t T) {
+  constexpr int kB = 4;
+  for (uint32_t j0 = 0; j0 < nv; j0 += kB * T) {
     uint32_t id[kB], dst[kB];
 #pragma unroll
     for (int i = 0; i < kB; ++i) {
-      const uint32_t j = j0 + (uint32_t)i * kWave + l;
+      const uint32_t j = j0 + (uint32_t)i * T + t;
       const uint32_t jj = j < nv ? j : 0u;
       id[i] = sid[jj];
       dst[i] = delta[(uint32_t)(skey[jj] >> shift) & (kRadixBins - 1)] + jj;
@@ -248,7 +276,7 @@ __device__ __forceinline__ void scatter_gather_out(const DJob& J, const KeyT* sk
     }
 #pragma unroll
     for (int i = 0; i < kB; ++i) {
-      if (j0 + (uint32_t)i * kWave + l >= nv) continue;
+      if (j0 + (uint32_t)i * T + t >= nv) continue;
       idx_out[dst[i]] = id[i];
       const uint64_t p = dst[i] - J.pbase;
 #pragma unroll
@@ -258,61 +286,61 @@ __device__ __forceinline__ void scatter_gather_out(const DJob& J, const KeyT* sk
   }
 }
 template <typename KeyT, bool GATHER>
-__global__ __launch_bounds__(kWave) void k_radix_scatter_lds(
+__global__ __launch_bounds__(kBlock) void k_radix_scatter_lds(
     const DJob* __restrict__ jobs, const uint2* __restrict__ chunks,
     const uint32_t* __restrict__ chunk_first, uint32_t n_chunks,
     const KeyT* __restrict__ keys_in, const uint32_t* __restrict__ idx_in,
     KeyT* __restrict__ keys_out, uint32_t* __restrict__ idx_out, int shift,
     const uint32_t* __restrict__ offs, const float* __restrict__ X, float* __restrict__ scoord) {
-  __shared__ uint32_t lrun[kRadixBins];
-  __shared__ uint32_t delta[kRadixBins];
+  __shared__ uint32_t wrun[kScatterWaves][kRadixBins];  // per wave and digit: count, then run start
+  __shared__ uint32_t lstart[kRadixBins];               // chunk-local start of each digit
+  __shared__ uint32_t delta[kRadixBins];                // global start - chunk-local start
+  __shared__ uint32_t wsum[kScatterWaves];
   __shared__ KeyT skey[kChunk];
   __shared__ uint32_t sid[kChunk];
   const int l = lane_id();
+  const uint32_t w = threadIdx.x / kWave;
+  // the chunk is the workgroup's: the early exit is uniform over it
   const uint32_t c = GATHER ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x;
   if (c >= n_chunks) return;
   const uint2 ch = chunks[c];
   const DJob& J = jobs[ch.x];
   const uint32_t c0 = chunk_first[ch.x];
   const uint32_t nch = chunk_first[ch.x + 1] - c0;
-  // digits 4l .. 4l+3: global start, chunk count, chunk-local start
   {
-    constexpr int kPer = kRadixBins / kWave;
-    uint32_t g[kPer], cnt[kPer], s = 0;
+    // digit d = thread: global start, chunk count, chunk-local start
+    const uint32_t d = threadIdx.x;
+    const uint64_t i = (uint64_t)kRadixBins * c0 + (uint64_t)d * nch + (c - c0);
+    const uint32_t g = offs[i];
+    const uint32_t cnt = offs[i + 1] - g;
+    uint32_t total;
+    const uint32_t ls = block_excl_scan(cnt, wsum, total);
+    lstart[d] = ls;
+    delta[d] = g - ls;
 #pragma unroll
-    for (int t = 0; t < kPer; ++t) {
-      const uint64_t i = (uint64_t)kRadixBins * c0 + (uint64_t)(kPer * l + t) * nch + (c - c0);
-      g[t] = offs[i];
-      cnt[t] = offs[i + 1] - g[t];
-      s += cnt[t];
-    }
-    uint32_t ls = wave_incl_scan(s) - s;
-#pragma unroll
-    for (int t = 0; t < kPer; ++t) {
-      lrun[kPer * l + t] = ls;
-      delta[kPer * l + t] = g[t] - ls;
-      ls += cnt[t];
-    }
+    for (int v = 0; v < kScatterWaves; ++v) wrun[v][d] = 0u;
   }
   const uint32_t nv = min((uint32_t)kChunk, J.n - ch.y);
-  KeyT kr[kChunkItems];
-  uint32_t ir[kChunkItems];
+  KeyT kr[kScatterRounds];
+  uint32_t ir[kScatterRounds];
 #pragma unroll
-  for (int i = 0; i < kChunkItems; ++i) {
-    const uint32_t k = ch.y + i * kWave + l;
-    kr[i] = 0;
-    ir[i] = 0;
-    if (k < J.n) {
-      kr[i] = keys_in[J.pbase + k];
-      ir[i] = idx_in[J.pbase + k];
+  for (int r = 0; r < kScatterRounds; ++r) {
+    const uint32_t e = (w * kScatterRounds + (uint32_t)r) * kWave + (uint32_t)l;
+    kr[r] = 0;
+    ir[r] = 0;
+    if (e < nv) {
+      kr[r] = keys_in[J.pbase + ch.y + e];
+      ir[r] = idx_in[J.pbase + ch.y + e];
     }
   }
-  wave_sync();
+  __syncthreads();  // counts cleared, digit starts written
   const uint64_t lt = lanemask_lt();
+  uint32_t* run = wrun[w];
+  uint32_t pos[kScatterRounds];
 #pragma unroll
-  for (int i = 0; i < kChunkItems; ++i) {
-    const bool valid = (uint32_t)(i * kWave + l) < nv;
-    const uint32_t dg = (uint32_t)(kr[i] >> shift) & (kRadixBins - 1);
+  for (int r = 0; r < kScatterRounds; ++r) {
+    const bool valid = (w * kScatterRounds + (uint32_t)r) * kWave + (uint32_t)l < nv;
+    const uint32_t dg = (uint32_t)(kr[r] >> shift) & (kRadixBins - 1);
     uint64_t match = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < kRadixBits; ++b) {
@@ -321,22 +349,42 @@ __global__ __launch_bounds__(kWave) void k_radix_scatter_lds(
       match &= bit ? bal : ~bal;
     }
     const uint32_t rank = __popcll(match & lt);
-    const uint32_t base = valid ? lrun[dg] : 0u;
+    const uint32_t base = valid ? run[dg] : 0u;
     wave_sync();
-    if (valid && rank == 0) lrun[dg] = base + (uint32_t)__popcll(match);
-    if (valid) {
-      skey[base + rank] = kr[i];
-      sid[base + rank] = ir[i];
-    }
+    if (valid && rank == 0) run[dg] = base + (uint32_t)__popcll(match);
+    pos[r] = base + rank;
     wave_sync();
   }
+  __syncthreads();  // every wave's counts are final
+  {
+    // runs of digit d: the chunk-local start, then the waves in order
+    const uint32_t d = threadIdx.x;
+    uint32_t acc = lstart[d];
+#pragma unroll
+    for (int v = 0; v < kScatterWaves; ++v) {
+      const uint32_t t = wrun[v][d];
+      wrun[v][d] = acc;
+      acc += t;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kScatterRounds; ++r) {
+    if ((w * kScatterRounds + (uint32_t)r) * kWave + (uint32_t)l < nv) {
+      const uint32_t dg = (uint32_t)(kr[r] >> shift) & (kRadixBins - 1);
+      const uint32_t p = run[dg] + pos[r];
+      skey[p] = kr[r];
+      sid[p] = ir[r];
+    }
+  }
+  __syncthreads();  // the chunk sits in LDS in digit order
   if constexpr (GATHER) {
     const uint32_t d = __builtin_amdgcn_readfirstlane(J.d);
-    if (d == 1) scatter_gather_out<1>(J, skey, sid, delta, nv, shift, X, keys_out, idx_out, scoord, l);
-    else if (d == 2) scatter_gather_out<2>(J, skey, sid, delta, nv, shift, X, keys_out, idx_out, scoord, l);
-    else scatter_gather_out<3>(J, skey, sid, delta, nv, shift, X, keys_out, idx_out, scoord, l);
+    if (d == 1) scatter_gather_out<1>(J, skey, sid, delta, nv, shift, X, keys_out, idx_out, scoord, threadIdx.x, kBlock);
+    else if (d == 2) scatter_gather_out<2>(J, skey, sid, delta, nv, shift, X, keys_out, idx_out, scoord, threadIdx.x, kBlock);
+    else scatter_gather_out<3>(J, skey, sid, delta, nv, shift, X, keys_out, idx_out, scoord, threadIdx.x, kBlock);
   } else {
-    for (uint32_t j = l; j < nv; j += kWave) {
+    for (uint32_t j = threadIdx.x; j < nv; j += kBlock) {
       const KeyT key = skey[j];
       const uint32_t dst = delta[(uint32_t)(key >> shift) & (kRadixBins - 1)] + j;
       idx_out[dst] = sid[j];
@@ -440,25 +488,6 @@ __global__ __launch_bounds__(kBlock) void k_major_keys(const DJob* __restrict__ 
 // kScanItems elements per thread, kBlock threads per block.
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kBlock * kScanItems;
-
-// Exclusive prefix of each thread's v over the workgroup (wave scans, then
-// the kBlock / kWave wave totals through LDS: one barrier) and the
-// workgroup's total.  The caller synchronises before wsum is written again.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t& total) {
-  const uint32_t incl = wave_incl_scan(v);
-  const uint32_t w = threadIdx.x / kWave;
-  if (lane_id() == kWave - 1) wsum[w] = incl;
-  __syncthreads();
-  uint32_t off = 0;
-  total = 0;
-#pragma unroll
-  for (uint32_t i = 0; i < (uint32_t)(kBlock / kWave); ++i) {
-    const uint32_t t = wsum[i];
-    off += i < w ? t : 0u;
-    total += t;
-  }
-  return off + incl - v;
-}
 
 __global__ __launch_bounds__(kBlock) void k_scan_blocks(const uint32_t* in, uint32_t* out,
                                                         uint64_t m, uint32_t* block_sums) {
@@ -997,7 +1026,7 @@ void launch_radix_scatter(hipStream_t s, bool wide, bool staged, const DJob* job
 #define ENT_SCATTER(KT, G)                                                                                     \
   do {                                                                                                         \
     if (staged)                                                                                                \
-      hipLaunchKernelGGL((k_radix_scatter_lds<KT, G>), dim3(n_chunks), dim3(kWave), 0, s, jobs, chunks,      \
+      hipLaunchKernelGGL((k_radix_scatter_lds<KT, G>), dim3(n_chunks), dim3(kBlock), 0, s, jobs, chunks,     \
                          chunk_first, n_chunks, (const KT*)kin, iin, (KT*)kout, iout, shift, offs, X, scoord); \
     else                                                                                                       \
       hipLaunchKernelGGL((k_radix_scatter<KT, G>), chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,   \
