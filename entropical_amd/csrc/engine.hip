@@ -686,7 +686,9 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     }
     if (!with_tiles) return;
     const uint2* tch = (const uint2*)d_tchunks.p;
-    launch_cell_starts(stream_, wide, dj, tch, NC, keys[cur], tcnt, (uint32_t*)d_cellstart.p);
+    // the tile-start flags go to the radix histogram buffer (kRadixBins
+    // words per chunk, free once the sort is done; the flags take kWave)
+    launch_cell_starts(stream_, wide, dj, tch, NC, keys[cur], tcnt, (uint32_t*)d_cellstart.p, hist);
     if (has_mom)
       launch_box_moments(stream_, dj, (const uint32_t*)d_mlist.p, mom_waves, scoord, (const uint32_t*)d_cellstart.p,
                          (float4*)d_moments.p);
@@ -694,7 +696,7 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
       launch_cell_moments2(stream_, dj, (const uint32_t*)d_mlist2.p, mom2_blocks, scoord,
                            (const uint32_t*)d_cellstart.p, (float*)d_moments2.p);
     launch_exclusive_scan(stream_, tcnt, tcnt, NC, scratch);
-    launch_tile_write(stream_, wide, dj, tch, NC, keys[cur], tcnt, (DTile*)d_tiles.p);
+    launch_tile_write(stream_, dj, tch, NC, hist, tcnt, (DTile*)d_tiles.p);
     launch_tile_ends(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
     if (pack) launch_tile_pack(stream_, dj, (DTile*)d_tiles.p, tcnt + NC, P);
   };

@@ -774,23 +774,6 @@ __device__ __forceinline__ bool tile_start(const DJob& J, KeyT a, KeyT b, uint32
   return (uint32_t)(a - ra * nb0) / J.group != (uint32_t)(b - rb * nb0) / J.group;
 }
 static_assert(kChunkItems <= 32, "tile-start flags are one 32-bit word per lane");
-// Tile-start flags of a chunk: bit i = point y + 64 i + l starts a tile.
-template <typename KeyT>
-__device__ __forceinline__ uint32_t chunk_tile_starts(const DJob& J, uint32_t y, const KeyT* keys, int l) {
-  constexpr int kHalf = kChunkItems / 2;
-  uint32_t bits = 0;
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    KeyT cur[kHalf], prev[kHalf];
-    load_key_pairs<kHalf>(J, y, h * kHalf, keys, l, cur, prev);
-#pragma unroll
-    for (int i = 0; i < kHalf; ++i) {
-      const uint32_t p = y + (uint32_t)(h * kHalf + i) * kWave + l;
-      if (p < J.n && tile_start(J, prev[i], cur[i], p)) bits |= 1u << (h * kHalf + i);
-    }
-  }
-  return bits;
-}
 }  // namespace
 
 // Cell-start tables (jobs with a table) and per-chunk tile counts (every
@@ -801,7 +784,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
                                                         const uint2* __restrict__ chunks,
                                                         uint32_t n_chunks, const KeyT* __restrict__ keys,
                                                         uint32_t* __restrict__ tile_cnt,
-                                                        uint32_t* __restrict__ start) {
+                                                        uint32_t* __restrict__ start,
+                                                        uint32_t* __restrict__ tbits) {
   const uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (c >= n_chunks) return;
   const uint2 ch = chunks[c];
@@ -810,6 +794,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
   const int l = lane_id();
   uint32_t* st = start + (tab ? J.cbase : 0);
   uint32_t tiles = 0;  // tile starts among the lane's points
+  uint32_t bits = 0;   // bit i: point ch.y + 64 i + lane starts a tile
   const KeyT* kk = keys + J.pbase;
   // Row-start tables index rows of boxes (key / nb0), cell-start tables cells
   // (the key itself).
@@ -834,7 +819,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
       const uint32_t p = p0 + (uint32_t)l;
       uint64_t first = 0, cnt = 0;
       if (p < J.n) {
-        if (tile_start(J, prevk[ii], curk[ii], p)) ++tiles;
+        if (tile_start(J, prevk[ii], curk[ii], p)) {
+          ++tiles;
+          bits |= 1u << (h * kHalf + ii);
+        }
         if (tab) {
           const uint64_t cur = unit(curk[ii]);
           first = p == 0 ? 0 : unit(prevk[ii]) + 1;
@@ -863,24 +851,23 @@ __global__ __launch_bounds__(kBlock) void k_cell_starts(const DJob* __restrict__
   }
   const uint32_t total = __builtin_amdgcn_readlane(wave_incl_scan(tiles), kWave - 1);
   if (l == 0) tile_cnt[c] = total;
+  tbits[(uint64_t)c * kWave + l] = bits;
 }
 
 
-template <typename KeyT>
 __global__ __launch_bounds__(kBlock) void k_tile_write(const DJob* __restrict__ jobs,
                                                        const uint2* __restrict__ chunks,
                                                        uint32_t n_chunks,
-                                                       const KeyT* __restrict__ keys,
+                                                       const uint32_t* __restrict__ tbits,
                                                        const uint32_t* __restrict__ offs,
                                                        DTile* __restrict__ tiles) {
   const uint32_t c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (c >= n_chunks) return;
   const uint2 ch = chunks[c];
-  const DJob& J = jobs[ch.x];
   const int l = lane_id();
   const uint64_t lt = lanemask_lt();
   uint32_t o = offs[c];
-  const uint32_t bits = chunk_tile_starts(J, ch.y, keys, l);
+  const uint32_t bits = tbits[(uint64_t)c * kWave + l];
 #pragma unroll
   for (int i = 0; i < kChunkItems; ++i) {
     const uint32_t p = ch.y + i * kWave + l;
@@ -1087,14 +1074,10 @@ void launch_gather_sorted(hipStream_t s, const DJob* jobs, const uint2* chunks, 
                      idx, scoord);
 }
 
-void launch_tile_write(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const void* keys, const uint32_t* offs, DTile* tiles) {
-  if (wide)
-    hipLaunchKernelGGL(k_tile_write<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint64_t*)keys, offs, tiles);
-  else
-    hipLaunchKernelGGL(k_tile_write<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint32_t*)keys, offs, tiles);
+void launch_tile_write(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
+                       const uint32_t* tbits, const uint32_t* offs, DTile* tiles) {
+  hipLaunchKernelGGL(k_tile_write, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks, n_chunks, tbits, offs,
+                     tiles);
 }
 
 void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
@@ -1176,13 +1159,13 @@ void launch_cell_moments2(hipStream_t s, const DJob* jobs, const uint32_t* mlist
 }
 
 void launch_cell_starts(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                        const void* keys, uint32_t* tile_cnt, uint32_t* start) {
+                        const void* keys, uint32_t* tile_cnt, uint32_t* start, uint32_t* tbits) {
   if (wide)
     hipLaunchKernelGGL(k_cell_starts<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint64_t*)keys, tile_cnt, start);
+                       n_chunks, (const uint64_t*)keys, tile_cnt, start, tbits);
   else
     hipLaunchKernelGGL(k_cell_starts<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, (const uint32_t*)keys, tile_cnt, start);
+                       n_chunks, (const uint32_t*)keys, tile_cnt, start, tbits);
 }
 
 }  // namespace ent

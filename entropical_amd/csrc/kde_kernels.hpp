@@ -18,7 +18,8 @@ void launch_radix_hist(hipStream_t s, bool wide, const DJob* jobs, const uint2* 
                        const uint32_t* chunk_first, uint32_t n_chunks, const void* keys, int shift,
                        uint32_t* hist);
 // scoord != nullptr: also write sorted coordinates and full cell keys (last
-// pass).  staged: LDS-staged variant (one wave per chunk, coalesced stores).
+// pass).  staged: LDS-staged variant (one workgroup of four waves per chunk,
+// coalesced stores).
 void launch_radix_scatter(hipStream_t s, bool wide, bool staged, const DJob* jobs, const uint2* chunks,
                           const uint32_t* chunk_first, uint32_t n_chunks, const void* kin,
                           const uint32_t* iin, void* kout, uint32_t* iout, int shift, const uint32_t* offs,
@@ -35,9 +36,10 @@ void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uin
 void launch_gather_sorted(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
                           const float* X, const uint32_t* idx, float* scoord);
 // Tile boundaries: every job.tq points, at row changes and between dim-0
-// groups (counted per chunk by launch_cell_starts).
-void launch_tile_write(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                       const void* keys, const uint32_t* offs, DTile* tiles);
+// groups (flagged and counted per chunk by launch_cell_starts: tbits holds
+// one 32-bit word per lane and chunk, offs the scanned counts).
+void launch_tile_write(hipStream_t s, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
+                       const uint32_t* tbits, const uint32_t* offs, DTile* tiles);
 void launch_tile_ends(hipStream_t s, const DJob* jobs, DTile* tiles, const uint32_t* total,
                       uint64_t max_tiles);
 // Packs runs of small tiles of one job for side-by-side processing (in place).
@@ -84,8 +86,10 @@ constexpr uint32_t kMomentBoxesPerWave = 4;
 void launch_cell_moments2(hipStream_t s, const DJob* jobs, const uint32_t* mlist, uint32_t max_blocks,
                           const float* scoord, const uint32_t* start, float* mom);
 // Cell-start tables of the jobs that have one, and the tile count of every
-// chunk (chunks in tile order) into tile_cnt, in one pass over the keys.
+// chunk (chunks in tile order) into tile_cnt, in one pass over the keys; the
+// tile-start flags into tbits (kWave words per chunk: bit i of lane l's word
+// is point 64 i + l of the chunk).
 void launch_cell_starts(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
-                        const void* keys, uint32_t* tile_cnt, uint32_t* start);
+                        const void* keys, uint32_t* tile_cnt, uint32_t* start, uint32_t* tbits);
 
 }  // namespace ent
