@@ -652,12 +652,13 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   // stable LSD passes over `np` digits of a chunked batch; the last one of
   // the main batch gathers the coordinates and writes full keys
   auto radix = [&](const DJob* jb, const uint2* cb, const uint32_t* cf, uint32_t nc, void** kk, uint32_t** ii,
-                   int np, float* sc_last) {
+                   int np, float* sc_last, bool first_counted) {
     const uint64_t hm = (uint64_t)kRadixBins * nc;
     int cur = 0;
     for (int pass = 0; pass < np; ++pass) {
       const int shift = pass * kRadixBits;
-      launch_radix_hist(stream_, wide, jb, cb, cf, nc, kk[cur], shift, hist);
+      // the first pass's digit counts may come with the keys (k_major_keys)
+      if (pass > 0 || !first_counted) launch_radix_hist(stream_, wide, jb, cb, cf, nc, kk[cur], shift, hist);
       launch_exclusive_scan(stream_, hist, hist, hm, scratch);
       launch_radix_scatter(stream_, wide, staged, jb, cb, cf, nc, kk[cur], ii[cur], kk[cur ^ 1], ii[cur ^ 1],
                            shift, hist, X, pass == np - 1 ? sc_last : nullptr);
@@ -675,13 +676,13 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
       uint32_t* bi[2] = {(uint32_t*)d_bidx[0].p, (uint32_t*)d_bidx[1].p};
       launch_cell_keys(stream_, wide, dbj, (const uint2*)d_bchunks.p, NCB, X, bk[0], bi[0]);
       const int bc = radix(dbj, (const uint2*)d_bchunks.p, (const uint32_t*)d_bchunk_first.p, NCB, bk, bi,
-                           npass0, nullptr);
+                           npass0, nullptr, false);
       launch_major_keys(stream_, wide, dj, dch, NC, (const uint64_t*)d_bbase.p, bi[bc], X, keys[0], idx[0],
-                        npass == 0 ? scoord : nullptr);
-      cur = radix(dj, dch, dcf, NC, keys, idx, npass, scoord);
+                        npass == 0 ? scoord : nullptr, dcf, npass > 0 ? hist : nullptr);
+      cur = radix(dj, dch, dcf, NC, keys, idx, npass, scoord, npass > 0);
     } else {
       launch_cell_keys(stream_, wide, dj, dch, NC, X, keys[0], idx[0]);
-      cur = radix(dj, dch, dcf, NC, keys, idx, npass, with_tiles ? scoord : nullptr);
+      cur = radix(dj, dch, dcf, NC, keys, idx, npass, with_tiles ? scoord : nullptr, false);
       if (with_tiles && npass == 0) launch_gather_sorted(stream_, dj, dch, NC, X, idx[cur], scoord);
     }
     if (!with_tiles) return;

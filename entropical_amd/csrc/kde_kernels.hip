@@ -408,7 +408,8 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter_lds(
 template <int D, bool FINAL, typename KeyT>
 __device__ __forceinline__ void major_keys_chunk(const DJob& J, uint32_t y, const uint32_t* __restrict__ bi,
                                                  const float* __restrict__ X, KeyT* __restrict__ keys,
-                                                 uint32_t* __restrict__ idx, float* __restrict__ scoord, int l) {
+                                                 uint32_t* __restrict__ idx, float* __restrict__ scoord, int l,
+                                                 uint32_t* bins) {
   constexpr int kB = 16;
   constexpr int E0 = FINAL ? 0 : 1;  // without coordinates only the dims above 0 are needed
   const uint32_t last = J.n - 1;
@@ -450,6 +451,7 @@ __device__ __forceinline__ void major_keys_chunk(const DJob& J, uint32_t y, cons
           r = box_of(v[i][1], J.mn[1], J.bs[1], J.nb[1]);
         }
         keys[J.pbase + k] = (KeyT)r;
+        if (bins) atomicAdd(&bins[(uint32_t)r & (kRadixBins - 1)], 1u);
       }
     }
   }
@@ -464,22 +466,40 @@ __global__ __launch_bounds__(kBlock) void k_major_keys(const DJob* __restrict__ 
                                                        const float* __restrict__ X,
                                                        KeyT* __restrict__ keys,
                                                        uint32_t* __restrict__ idx,
-                                                       float* __restrict__ scoord) {
-  const uint32_t c = xcd_order(blockIdx.x, gridDim.x) * (kBlock / kWave) + threadIdx.x / kWave;
+                                                       float* __restrict__ scoord,
+                                                       const uint32_t* __restrict__ chunk_first,
+                                                       uint32_t* __restrict__ hist) {
+  // with hist: the digit counts of the first radix pass over the keys written
+  // here (digit = key & 0xFF), in k_radix_hist's layout
+  __shared__ uint32_t bins[kBlock / kWave][kRadixBins];
+  const int w = threadIdx.x / kWave;
+  const int l = lane_id();
+  if (hist) {
+    for (int b = l; b < kRadixBins; b += kWave) bins[w][b] = 0u;
+    wave_sync();
+  }
+  const uint32_t c = xcd_order(blockIdx.x, gridDim.x) * (kBlock / kWave) + (uint32_t)w;
   if (c >= n_chunks) return;
   const uint2 ch = chunks[c];
   const DJob J = jobs[ch.x];
   const uint32_t* bi = bidx + bbase[ch.x];
-  const int l = lane_id();
   const uint32_t d = __builtin_amdgcn_readfirstlane(J.d);
+  uint32_t* wb = hist ? bins[w] : nullptr;
   if (scoord) {
-    if (d == 1) major_keys_chunk<1, true>(J, ch.y, bi, X, keys, idx, scoord, l);
-    else if (d == 2) major_keys_chunk<2, true>(J, ch.y, bi, X, keys, idx, scoord, l);
-    else major_keys_chunk<3, true>(J, ch.y, bi, X, keys, idx, scoord, l);
+    if (d == 1) major_keys_chunk<1, true>(J, ch.y, bi, X, keys, idx, scoord, l, nullptr);
+    else if (d == 2) major_keys_chunk<2, true>(J, ch.y, bi, X, keys, idx, scoord, l, nullptr);
+    else major_keys_chunk<3, true>(J, ch.y, bi, X, keys, idx, scoord, l, nullptr);
   } else {
-    if (d == 1) major_keys_chunk<1, false>(J, ch.y, bi, X, keys, idx, scoord, l);
-    else if (d == 2) major_keys_chunk<2, false>(J, ch.y, bi, X, keys, idx, scoord, l);
-    else major_keys_chunk<3, false>(J, ch.y, bi, X, keys, idx, scoord, l);
+    if (d == 1) major_keys_chunk<1, false>(J, ch.y, bi, X, keys, idx, scoord, l, wb);
+    else if (d == 2) major_keys_chunk<2, false>(J, ch.y, bi, X, keys, idx, scoord, l, wb);
+    else major_keys_chunk<3, false>(J, ch.y, bi, X, keys, idx, scoord, l, wb);
+  }
+  if (hist && !scoord) {
+    wave_sync();
+    const uint32_t c0 = chunk_first[ch.x];
+    const uint32_t nch = chunk_first[ch.x + 1] - c0;
+    for (int b = l; b < kRadixBins; b += kWave)
+      hist[(uint64_t)kRadixBins * c0 + (uint64_t)b * nch + (c - c0)] = bins[w][b];
   }
 }
 
@@ -1031,13 +1051,13 @@ void launch_radix_scatter(hipStream_t s, bool wide, bool staged, const DJob* job
 
 void launch_major_keys(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
                        const uint64_t* bbase, const uint32_t* bidx, const float* X, void* keys,
-                       uint32_t* idx, float* scoord) {
+                       uint32_t* idx, float* scoord, const uint32_t* chunk_first, uint32_t* hist) {
   if (wide)
     hipLaunchKernelGGL(k_major_keys<uint64_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, bbase, bidx, X, (uint64_t*)keys, idx, scoord);
+                       n_chunks, bbase, bidx, X, (uint64_t*)keys, idx, scoord, chunk_first, hist);
   else
     hipLaunchKernelGGL(k_major_keys<uint32_t>, chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,
-                       n_chunks, bbase, bidx, X, (uint32_t*)keys, idx, scoord);
+                       n_chunks, bbase, bidx, X, (uint32_t*)keys, idx, scoord, chunk_first, hist);
 }
 
 uint64_t scan_scratch_elems(uint64_t m) {

@@ -26,10 +26,12 @@ void launch_radix_scatter(hipStream_t s, bool wide, bool staged, const DJob* job
                           const float* X, float* scoord);
 // Two-stage sort, stage 2 input: job order from its base's sorted points
 // (bidx + bbase[job]); key = box key of dims 1.. (or, with scoord, the full
-// key plus the sorted coordinates).
+// key plus the sorted coordinates).  hist != nullptr (without scoord): also
+// the per-chunk digit counts of the first radix pass over these keys (shift
+// 0), as launch_radix_hist writes them.
 void launch_major_keys(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
                        const uint64_t* bbase, const uint32_t* bidx, const float* X, void* keys,
-                       uint32_t* idx, float* scoord);
+                       uint32_t* idx, float* scoord, const uint32_t* chunk_first, uint32_t* hist);
 uint64_t scan_scratch_elems(uint64_t m);
 void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t m,
                            uint32_t* scratch);

@@ -15,50 +15,64 @@ constexpr int kRBlock = 256;
 
 __device__ __forceinline__ double inv0(double x) { return x == 0.0 ? x : 1.0 / x; }  // tools.hxx:45-53
 
-// One term of the selected formula (FP64 throughout; log of 0 -> 0 as
-// select_log does, tools.cpp:18-37).
+// One term of formula OP (FP64 throughout; log of 0 -> 0 as select_log
+// does, tools.cpp:18-37).
+template <uint32_t OP>
 __device__ __forceinline__ double term(const RedItem& it, uint64_t k, const double* __restrict__ D) {
   const double a = D[it.off[0] + k];
-  switch (it.op) {
-    case kRedMI: {  // mi.cpp:53-56: p_ij * log(p_ij / p_i / p_j), one division
-      const double r = a / (D[it.off[1] + k] * D[it.off[2] + k]);
-      return r == 0.0 ? 0.0 : a * log(r) * it.scale;
-    }
-    case kRedLogMean:  // sum of ln p (entropy = -mean)
-      return log(a);
-    case kRedTE: {  // transs.cpp:77-100: p3*pref*log(p3*p1*inv(p2)*inv(ptau))
-      if (!(a > 0.0)) return 0.0;
-      // inv(p2) * inv(ptau) as one division (0 when either is 0, as inv does)
-      const double den = D[it.off[2] + k] * D[it.off[3] + k];
-      const double r = den == 0.0 ? 0.0 : a * D[it.off[1] + k] / den;
-      if (r == 0.0) return 0.0;
-      const double te = a * it.pref * log(r) * it.scale;
-      return (isfinite(te) && a <= 1.0) ? te : 0.0;
-    }
-    case kRedAutoEnt: {  // autoent.cpp:52-56: p2*h*h*log(p1*inv(p2*h)), p2 = a
-      if (!(a > 0.0)) return 0.0;
-      const double r = D[it.off[1] + k] * inv0(a * it.pref);
-      return r == 0.0 ? 0.0 : a * it.pref * it.pref * log(r) * it.scale;
-    }
-    case kRedLoo: {  // p = density incl. self pair, divided by n (n_rows == n)
-      const double nd = (double)it.n;
-      const double loo = (nd * a - it.pref) / (nd - 1.0);
-      return log(loo > it.scale ? loo : it.scale);
-    }
-    case kRedSum:
-      return a * it.scale;
-    case kRedKL: {  // select_log: log of a zero ratio is 0
-      const double qa = a * it.pref;
-      const double r = qa / (D[it.off[1] + k] * it.pref2);
-      return r == 0.0 ? 0.0 : qa * log(r) * it.scale;
-    }
-    case kRedNegs: {
-      const double q = a * it.pref;
-      return q == 0.0 ? 0.0 : log(q) * q * it.scale;
-    }
-    default:
-      return 0.0;
+  if constexpr (OP == kRedMI) {  // mi.cpp:53-56: p_ij * log(p_ij / p_i / p_j), one division
+    const double r = a / (D[it.off[1] + k] * D[it.off[2] + k]);
+    return r == 0.0 ? 0.0 : a * log(r) * it.scale;
+  } else if constexpr (OP == kRedLogMean) {  // sum of ln p (entropy = -mean)
+    return log(a);
+  } else if constexpr (OP == kRedTE) {  // transs.cpp:77-100: p3*pref*log(p3*p1*inv(p2)*inv(ptau))
+    if (!(a > 0.0)) return 0.0;
+    // inv(p2) * inv(ptau) as one division (0 when either is 0, as inv does)
+    const double den = D[it.off[2] + k] * D[it.off[3] + k];
+    const double r = den == 0.0 ? 0.0 : a * D[it.off[1] + k] / den;
+    if (r == 0.0) return 0.0;
+    const double te = a * it.pref * log(r) * it.scale;
+    return (isfinite(te) && a <= 1.0) ? te : 0.0;
+  } else if constexpr (OP == kRedAutoEnt) {  // autoent.cpp:52-56: p2*h*h*log(p1*inv(p2*h)), p2 = a
+    if (!(a > 0.0)) return 0.0;
+    const double r = D[it.off[1] + k] * inv0(a * it.pref);
+    return r == 0.0 ? 0.0 : a * it.pref * it.pref * log(r) * it.scale;
+  } else if constexpr (OP == kRedLoo) {  // p = density incl. self pair, divided by n (n_rows == n)
+    const double nd = (double)it.n;
+    const double loo = (nd * a - it.pref) / (nd - 1.0);
+    return log(loo > it.scale ? loo : it.scale);
+  } else if constexpr (OP == kRedSum) {
+    return a * it.scale;
+  } else if constexpr (OP == kRedKL) {  // select_log: log of a zero ratio is 0
+    const double qa = a * it.pref;
+    const double r = qa / (D[it.off[1] + k] * it.pref2);
+    return r == 0.0 ? 0.0 : qa * log(r) * it.scale;
+  } else {  // kRedNegs
+    const double q = a * it.pref;
+    return q == 0.0 ? 0.0 : log(q) * q * it.scale;
   }
+}
+
+// Terms k0 + t, k0 + t + kRBlock, ... below k1 of thread t, summed in that
+// order; four terms at a time are computed before they are added, so their
+// loads are in flight together.
+template <uint32_t OP>
+__device__ __forceinline__ double sum_terms(const RedItem& it, const double* __restrict__ D, uint64_t k0,
+                                            uint64_t k1) {
+  double acc = 0.0;
+  uint64_t k = k0 + threadIdx.x;
+  for (; k + 3 * (uint64_t)kRBlock < k1; k += 4 * (uint64_t)kRBlock) {
+    const double t0 = term<OP>(it, k, D);
+    const double t1 = term<OP>(it, k + kRBlock, D);
+    const double t2 = term<OP>(it, k + 2 * kRBlock, D);
+    const double t3 = term<OP>(it, k + 3 * kRBlock, D);
+    acc += t0;
+    acc += t1;
+    acc += t2;
+    acc += t3;
+  }
+  for (; k < k1; k += kRBlock) acc += term<OP>(it, k, D);
+  return acc;
 }
 
 __global__ __launch_bounds__(kRBlock) void k_reduce_partial(const RedItem* __restrict__ items,
@@ -73,7 +87,17 @@ __global__ __launch_bounds__(kRBlock) void k_reduce_partial(const RedItem* __res
   const uint64_t k0 = (uint64_t)blockIdx.x * per;
   const uint64_t k1 = k0 + per < n ? k0 + per : n;
   double acc = 0.0;
-  for (uint64_t k = k0 + threadIdx.x; k < k1; k += kRBlock) acc += term(it, k, D);
+  switch (it.op) {  // uniform over the workgroup
+    case kRedMI: acc = sum_terms<kRedMI>(it, D, k0, k1); break;
+    case kRedLogMean: acc = sum_terms<kRedLogMean>(it, D, k0, k1); break;
+    case kRedTE: acc = sum_terms<kRedTE>(it, D, k0, k1); break;
+    case kRedAutoEnt: acc = sum_terms<kRedAutoEnt>(it, D, k0, k1); break;
+    case kRedLoo: acc = sum_terms<kRedLoo>(it, D, k0, k1); break;
+    case kRedSum: acc = sum_terms<kRedSum>(it, D, k0, k1); break;
+    case kRedKL: acc = sum_terms<kRedKL>(it, D, k0, k1); break;
+    case kRedNegs: acc = sum_terms<kRedNegs>(it, D, k0, k1); break;
+    default: break;
+  }
   s[threadIdx.x] = acc;
   __syncthreads();
   for (int o = kRBlock / 2; o > 0; o >>= 1) {
