@@ -345,13 +345,20 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
     }
     D.cells = cells;
     D.cbase = table_total;
-    if (cells + 1 <= kTableCellsPerPoint * (uint64_t)D.n + 4096) {
+    const uint64_t table_cap = kTableCellsPerPoint * (uint64_t)D.n + 4096;
+    if (cells + 1 <= table_cap) {
       table_total += cells + 1;
-    } else {
+    } else if (cells / D.nb[0] + 1 <= table_cap) {
       // one entry per row of boxes: a binary search over one row's points
       // instead of the job's
       D.flags |= kJobRowTable;
       table_total += cells / D.nb[0] + 1;
+    } else {
+      // more rows than the table bound (sparse heavy-tailed 3-D data with
+      // narrow bandwidths): no table, each range is a binary search over the
+      // job's keys
+      D.flags |= kJobRowTable;
+      D.cbase = kNoTable;
     }
     max_key = std::max<uint64_t>(max_key, cells - 1);
   }

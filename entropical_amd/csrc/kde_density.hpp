@@ -43,11 +43,17 @@ __device__ __forceinline__ uint32_t lower_bound_key(const KeyT* a, uint32_t n, u
 
 // Sorted positions [a, b) of the points with keys klo..khi, which lie in one
 // row of boxes (row = key / nb0): read from the job's cell-start table, or
-// found by binary search within the row that its row-start table locates.
+// found by binary search within the row that its row-start table locates,
+// or (a job with neither table) within the job.
 template <typename KeyT>
 __device__ __forceinline__ void key_range(const DJob& J, const uint32_t* __restrict__ cstart,
                                           const KeyT* __restrict__ kk, uint64_t row, uint64_t klo, uint64_t khi,
                                           uint32_t& a, uint32_t& b) {
+  if (J.cbase == kNoTable) {
+    a = lower_bound_key(kk, J.n, klo);
+    b = a + lower_bound_key(kk + a, J.n - a, khi + 1);
+    return;
+  }
   const uint32_t* t = cstart + J.cbase;
   if (J.flags & kJobRowTable) {
     const uint32_t r0 = t[row], r1 = t[row + 1];
@@ -59,8 +65,16 @@ __device__ __forceinline__ void key_range(const DJob& J, const uint32_t* __restr
   }
 }
 // Sorted positions [a, b) of the rows row_a..row_b.
-__device__ __forceinline__ void rows_range(const DJob& J, const uint32_t* __restrict__ cstart, uint64_t row_a,
-                                           uint64_t row_b, uint32_t& a, uint32_t& b) {
+template <typename KeyT>
+__device__ __forceinline__ void rows_range(const DJob& J, const uint32_t* __restrict__ cstart,
+                                           const KeyT* __restrict__ kk, uint64_t row_a, uint64_t row_b,
+                                           uint32_t& a, uint32_t& b) {
+  if (J.cbase == kNoTable) {
+    const uint64_t nb0 = J.nb[0];
+    a = lower_bound_key(kk, J.n, row_a * nb0);
+    b = a + lower_bound_key(kk + a, J.n - a, (row_b + 1) * nb0);
+    return;
+  }
   const uint32_t* t = cstart + J.cbase;
   if (J.flags & kJobRowTable) {
     a = t[row_a];
@@ -508,7 +522,7 @@ __device__ uint32_t process_tile(const DJob& J, const DTile& T, const float* __r
       uint64_t row_b = (uint64_t)hi[2] * J.nb[1] + hi[1];
       if (l == 0) {
         uint32_t a, b;
-        rows_range(J, cstart, row_a, row_b, a, b);
+        rows_range(J, cstart, keys + J.pbase, row_a, row_b, a, b);
         my_lo = a;
         my_len = b - a;
       }
@@ -696,7 +710,7 @@ __device__ uint64_t process_pack(const DJob& J, const DTile& T, const float* __r
       const uint64_t row_a = (uint64_t)lo[2] * J.nb[1] + lo[1];
       const uint64_t row_b = (uint64_t)hi[2] * J.nb[1] + hi[1];
       uint32_t a, b;
-      rows_range(J, cstart, row_a, row_b, a, b);
+      rows_range(J, cstart, kk, row_a, row_b, a, b);
       my_lo = a;
       my_len = b - a;
     }
@@ -849,7 +863,7 @@ __device__ void process_subt(const DJob& J, const DTile& T, bool packed, const f
     if (rows_fit && (uint32_t)j < n1) {
       key_range(J, cstart, kk, r, (uint64_t)r * J.nb[0] + lo[0], (uint64_t)r * J.nb[0] + hi[0], ra, rb);
     } else if (nq != 0 && !rows_fit && j == 0) {
-      rows_range(J, cstart, lo[1], hi[1], ra, rb);
+      rows_range(J, cstart, kk, lo[1], hi[1], ra, rb);
     }
   } else {
     auto row_of = [&](uint32_t i) { return (uint64_t)(lo[2] + i / n1) * J.nb[1] + lo[1] + i % n1; };
@@ -863,7 +877,7 @@ __device__ void process_subt(const DJob& J, const DTile& T, bool packed, const f
         key_range(J, cstart, kk, row, row * J.nb[0] + lo[0], row * J.nb[0] + hi[0], ra2, rb2);
       }
     } else if (nq != 0 && j == 0) {
-      rows_range(J, cstart, (uint64_t)lo[2] * J.nb[1] + lo[1], (uint64_t)hi[2] * J.nb[1] + hi[1], ra, rb);
+      rows_range(J, cstart, kk, (uint64_t)lo[2] * J.nb[1] + lo[1], (uint64_t)hi[2] * J.nb[1] + hi[1], ra, rb);
     }
   }
 

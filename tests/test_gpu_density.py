@@ -443,6 +443,33 @@ def test_moment_tables_over_cap_fall_back(ctx, orc, monkeypatch):
     assert st0["pairs"] == st1["pairs"]
 
 
+@pytest.mark.parametrize("tuning", [(1, 32, 2, 64), (2, 16, 1, 16), (1, 32, 2, 64, 16), (1, 32, 1, 64, 32)])
+def test_jobs_without_tables(ctx, orc, tuning):
+    """Heavy-tailed data with narrow bandwidths have more rows of boxes than
+    the table bound (8 entries per point + 4096): such jobs get neither a
+    cell-start nor a row-start table, and every candidate range is a binary
+    search over the job's keys.  2-D and 3-D jobs, whole tiles and
+    sub-tiles, within 1e-5 of FP64."""
+    from entropical_amd import combined_densities
+
+    r = np.random.default_rng(173)
+    n = 3000
+    x = (r.standard_t(1.5, size=(3, n)) * 3.0).astype(np.float32)
+    m0, mr = tuning[1], tuning[2]
+    try:
+        ctx.set_tuning(*tuning)
+        for cols, h in (([0, 1], [0.05, 0.01]), ([0, 1, 2], [0.05, 0.02, 0.02])):
+            # the engine's box counts (ceil(range / (h / m)) per dim): the
+            # rows of boxes exceed the table bound, so the job has no table
+            nb = [int(np.ceil(np.float32(x[c].max() - x[c].min()) / np.float32(np.float32(hh) / (m0 if e == 0 else mr))))
+                  for e, (c, hh) in enumerate(zip(cols, h))]
+            assert int(np.prod(nb[1:])) + 1 > 8 * n + 4096, nb
+            got = combined_densities(x, n, cols, h, ctx=ctx)
+            assert rel_err(got, orc.density_fp64_window(x, n, cols, h)) < RTOL, (cols, h)
+    finally:
+        ctx.set_tuning()
+
+
 @pytest.mark.parametrize("tuning", [(2, 32, 1, 64), (1, 32, 2, 64)])
 def test_wide_sparse_tiles(ctx, orc, tuning):
     """Tiles in sparse stretches span up to a 64-bandwidth dim-0 group.  The
