@@ -658,6 +658,9 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
   uint32_t* tcnt = (uint32_t*)d_tile_cnt.p;
   // stable LSD passes over `np` digits of a chunked batch; the last one of
   // the main batch gathers the coordinates and writes full keys
+  const uint32_t* bidx_final = nullptr;  // the bases' sorted order (shared sorts)
+  // first_counted: pass 0's digit counts and keys come from k_major_keys,
+  // and its input order is each job's base order (bidx + bbase[job])
   auto radix = [&](const DJob* jb, const uint2* cb, const uint32_t* cf, uint32_t nc, void** kk, uint32_t** ii,
                    int np, float* sc_last, bool first_counted) {
     const uint64_t hm = (uint64_t)kRadixBins * nc;
@@ -667,8 +670,10 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
       // the first pass's digit counts may come with the keys (k_major_keys)
       if (pass > 0 || !first_counted) launch_radix_hist(stream_, wide, jb, cb, cf, nc, kk[cur], shift, hist);
       launch_exclusive_scan(stream_, hist, hist, hm, scratch);
-      launch_radix_scatter(stream_, wide, staged, jb, cb, cf, nc, kk[cur], ii[cur], kk[cur ^ 1], ii[cur ^ 1],
-                           shift, hist, X, pass == np - 1 ? sc_last : nullptr);
+      const bool from_base = pass == 0 && first_counted;
+      launch_radix_scatter(stream_, wide, staged, jb, cb, cf, nc, kk[cur], from_base ? nullptr : ii[cur],
+                           kk[cur ^ 1], ii[cur ^ 1], shift, hist, X, pass == np - 1 ? sc_last : nullptr,
+                           from_base ? (const uint64_t*)d_bbase.p : nullptr, from_base ? bidx_final : nullptr);
       cur ^= 1;
     }
     return cur;
@@ -684,8 +689,10 @@ int Engine::plan_and_run_cell(const float* X, const std::vector<JobSpec>& jobs, 
       launch_cell_keys(stream_, wide, dbj, (const uint2*)d_bchunks.p, NCB, X, bk[0], bi[0]);
       const int bc = radix(dbj, (const uint2*)d_bchunks.p, (const uint32_t*)d_bchunk_first.p, NCB, bk, bi,
                            npass0, nullptr, false);
-      launch_major_keys(stream_, wide, dj, dch, NC, (const uint64_t*)d_bbase.p, bi[bc], X, keys[0], idx[0],
-                        npass == 0 ? scoord : nullptr, dcf, npass > 0 ? hist : nullptr);
+      bidx_final = bi[bc];
+      launch_major_keys(stream_, wide, dj, dch, NC, (const uint64_t*)d_bbase.p, bi[bc], X, keys[0],
+                        npass == 0 ? idx[0] : nullptr, npass == 0 ? scoord : nullptr, dcf,
+                        npass > 0 ? hist : nullptr);
       cur = radix(dj, dch, dcf, NC, keys, idx, npass, scoord, npass > 0);
     } else {
       launch_cell_keys(stream_, wide, dj, dch, NC, X, keys[0], idx[0]);

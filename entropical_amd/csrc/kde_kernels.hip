@@ -156,7 +156,8 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const uint32_t* __restrict__ chunk_first, uint32_t n_chunks,
     const KeyT* __restrict__ keys_in, const uint32_t* __restrict__ idx_in,
     KeyT* __restrict__ keys_out, uint32_t* __restrict__ idx_out, int shift,
-    const uint32_t* __restrict__ offs, const float* __restrict__ X, float* __restrict__ scoord) {
+    const uint32_t* __restrict__ offs, const float* __restrict__ X, float* __restrict__ scoord,
+    const uint64_t* __restrict__ bbase, const uint32_t* __restrict__ bidx) {
   __shared__ uint32_t run[kBlock / kWave][kRadixBins];
   const int w = threadIdx.x / kWave;
   const int l = lane_id();
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     uint32_t id = 0, dg = 0;
     if (valid) {
       key = keys_in[J.pbase + k];
-      id = idx_in[J.pbase + k];
+      id = idx_in ? idx_in[J.pbase + k] : bidx[bbase[ch.x] + k];
       dg = (uint32_t)(key >> shift) & (kRadixBins - 1);
     }
     uint64_t match = __ballot(valid);
@@ -291,7 +292,8 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter_lds(
     const uint32_t* __restrict__ chunk_first, uint32_t n_chunks,
     const KeyT* __restrict__ keys_in, const uint32_t* __restrict__ idx_in,
     KeyT* __restrict__ keys_out, uint32_t* __restrict__ idx_out, int shift,
-    const uint32_t* __restrict__ offs, const float* __restrict__ X, float* __restrict__ scoord) {
+    const uint32_t* __restrict__ offs, const float* __restrict__ X, float* __restrict__ scoord,
+    const uint64_t* __restrict__ bbase, const uint32_t* __restrict__ bidx) {
   __shared__ uint32_t wrun[kScatterWaves][kRadixBins];  // per wave and digit: count, then run start
   __shared__ uint32_t lstart[kRadixBins];               // chunk-local start of each digit
   __shared__ uint32_t delta[kRadixBins];                // global start - chunk-local start
@@ -321,6 +323,9 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter_lds(
     for (int v = 0; v < kScatterWaves; ++v) wrun[v][d] = 0u;
   }
   const uint32_t nv = min((uint32_t)kChunk, J.n - ch.y);
+  // the permutation so far: the previous pass's, or (idx_in == nullptr, the
+  // first pass over keys from k_major_keys) the job's base order itself
+  const uint32_t* isrc = idx_in ? idx_in + J.pbase : bidx + bbase[ch.x];
   KeyT kr[kScatterRounds];
   uint32_t ir[kScatterRounds];
 #pragma unroll
@@ -330,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter_lds(
     ir[r] = 0;
     if (e < nv) {
       kr[r] = keys_in[J.pbase + ch.y + e];
-      ir[r] = idx_in[J.pbase + ch.y + e];
+      ir[r] = isrc[ch.y + e];
     }
   }
   __syncthreads();  // counts cleared, digit starts written
@@ -435,7 +440,7 @@ __device__ __forceinline__ void major_keys_chunk(const DJob& J, uint32_t y, cons
     for (int i = 0; i < kB; ++i) {
       const uint32_t k = y + (uint32_t)(i0 + i) * kWave + l;
       if (k >= J.n) continue;
-      idx[J.pbase + k] = id[i];
+      if (FINAL || idx) idx[J.pbase + k] = id[i];
       if constexpr (FINAL) {
 #pragma unroll
         for (int e = 0; e < D; ++e) scoord[3 * J.pbase + (uint64_t)e * J.n + k] = v[i][e];
@@ -1028,16 +1033,18 @@ void launch_radix_hist(hipStream_t s, bool wide, const DJob* jobs, const uint2* 
 void launch_radix_scatter(hipStream_t s, bool wide, bool staged, const DJob* jobs, const uint2* chunks,
                           const uint32_t* chunk_first, uint32_t n_chunks, const void* kin,
                           const uint32_t* iin, void* kout, uint32_t* iout, int shift, const uint32_t* offs,
-                          const float* X, float* scoord) {
+                          const float* X, float* scoord, const uint64_t* bbase, const uint32_t* bidx) {
   const bool gather = scoord != nullptr;
 #define ENT_SCATTER(KT, G)                                                                                     \
   do {                                                                                                         \
     if (staged)                                                                                                \
       hipLaunchKernelGGL((k_radix_scatter_lds<KT, G>), dim3(n_chunks), dim3(kBlock), 0, s, jobs, chunks,     \
-                         chunk_first, n_chunks, (const KT*)kin, iin, (KT*)kout, iout, shift, offs, X, scoord); \
+                         chunk_first, n_chunks, (const KT*)kin, iin, (KT*)kout, iout, shift, offs, X, scoord, \
+                         bbase, bidx);                                                                 \
     else                                                                                                       \
       hipLaunchKernelGGL((k_radix_scatter<KT, G>), chunk_grid(n_chunks), dim3(kBlock), 0, s, jobs, chunks,   \
-                         chunk_first, n_chunks, (const KT*)kin, iin, (KT*)kout, iout, shift, offs, X, scoord); \
+                         chunk_first, n_chunks, (const KT*)kin, iin, (KT*)kout, iout, shift, offs, X, scoord, \
+                         bbase, bidx);                                                                 \
   } while (0)
   if (wide) {
     if (gather) ENT_SCATTER(uint64_t, true);

@@ -20,13 +20,17 @@ void launch_radix_hist(hipStream_t s, bool wide, const DJob* jobs, const uint2* 
 // scoord != nullptr: also write sorted coordinates and full cell keys (last
 // pass).  staged: LDS-staged variant (one workgroup of four waves per chunk,
 // coalesced stores).
+// iin == nullptr: the input order is each job's base order, bidx + bbase[job]
+// (the first pass after launch_major_keys, which then writes no permutation).
 void launch_radix_scatter(hipStream_t s, bool wide, bool staged, const DJob* jobs, const uint2* chunks,
                           const uint32_t* chunk_first, uint32_t n_chunks, const void* kin,
                           const uint32_t* iin, void* kout, uint32_t* iout, int shift, const uint32_t* offs,
-                          const float* X, float* scoord);
+                          const float* X, float* scoord, const uint64_t* bbase = nullptr,
+                          const uint32_t* bidx = nullptr);
 // Two-stage sort, stage 2 input: job order from its base's sorted points
 // (bidx + bbase[job]); key = box key of dims 1.. (or, with scoord, the full
-// key plus the sorted coordinates).  hist != nullptr (without scoord): also
+// key plus the sorted coordinates; idx == nullptr without scoord: no
+// permutation, the next pass reads the base order).  hist != nullptr (without scoord): also
 // the per-chunk digit counts of the first radix pass over these keys (shift
 // 0), as launch_radix_hist writes them.
 void launch_major_keys(hipStream_t s, bool wide, const DJob* jobs, const uint2* chunks, uint32_t n_chunks,
